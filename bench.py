@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the fused ZSC step on one MI355X.
+
+Workload: a 512^3 periodic box with 128 bubbles (R ~ U[8,16], seed 2) and the
+ZSC physics of lbm_bench_gpu_amd.configs -- FP32 populations for the headline
+value, plus the same box with bf16 and fp16 deviation storage.  A step is one
+reference time step (update, collide, stream), which is one launch of the
+fused kernel.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  `value` is MLUPS = cells * steps / wall time of the
+timed steps.  `roofline` prices the fused kernel at its algorithmic bytes per
+cell (26 populations read + written once) over its average launch duration
+from HIP events on the launch stream.  `cpu_baseline` times the FP64 oracle
+(the CPU restatement used as the parity checker) on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512, help="cube edge (default: 512)")
+    ap.add_argument("--variants", default="bf16,fp16",
+                    help="extra storage precisions timed beside fp32")
+    ap.add_argument("--ty", type=int, default=0, help="fused tile rows (0: library default)")
+    ap.add_argument("--zchunk", type=int, default=-1, help="fused z chunk (-1: library default)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU work of the oracle baseline (0 disables)")
+    return ap.parse_args()
+
+
+def time_precision(prec, w, params, steps, warmup, ty, zchunk):
+    import torch
+    from lbm_bench_gpu_amd import zsc
+    nx, ny, nz = w.dims
+    ctx = zsc.Context(nx, ny, nz, precision=prec)
+    try:
+        if ty > 0 or zchunk >= 0:
+            ctx.set_fused_config(ty if ty > 0 else 4, max(zchunk, 0) if zchunk >= 0 else 64)
+        ctx.set_params(params)
+        ctx.state_init_bubbles(w.bubbles)
+        ctx.step(warmup + 1)  # the first step also runs the A-lattice prologue
+        ctx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev_ms = ctx.step_timed(steps)  # hipEvents on the launch stream around the launches
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    cells = nx * ny * nz
+    bpc = zsc.bytes_per_cell(prec)
+    per_launch = ev_ms / 1e3 / steps
+    achieved = bpc * cells / per_launch / 1e9
+    return {
+        "mlups": cells * steps / wall / 1e6,
+        "ms_per_step": wall * 1e3 / steps,
+        "kernel_ms": per_launch * 1e3,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_cell": bpc},
+        "sum_phi": st["sum_phi"],
+        "finite": bool(np.isfinite(st["sum_phi"])),
+    }
+
+
+def cpu_baseline(seconds: float):
+    """FP64 oracle on a 128^3 box of the same physics (4 bubbles), timed on the
+    host's cores for about `seconds` of work."""
+    import oracle as O
+    from lbm_bench_gpu_amd import configs, zsc
+    dims = (128, 128, 128)
+    b = zsc.generate_bubbles(2, 4, 8, 16, 4.0, 4.0, dims)
+    p = zsc.derive_params(configs.physics(*dims), float(b[0, 3]))
+    c = O.ZoCoef()
+    for k, _ in O.ZoCoef._fields_:
+        setattr(c, k, p.values[k])
+    phi0 = O.init_phi((128, 128, 128), b, p.phi_star, p.width)
+    st = O.State((128, 128, 128), c).init(phi0)
+    t0 = time.perf_counter()
+    st.run(1)
+    one = time.perf_counter() - t0
+    k = max(1, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    st.run(k)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(128 ** 3 * k / dt / 1e6, 3), "unit": "MLUPS", "cores": cores,
+            "kind": "port",
+            "sample": f"FP64 oracle (C, OpenMP), 128^3 box, 4 bubbles, {k} steps, {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    from lbm_bench_gpu_amd import configs
+    from lbm_bench_gpu_amd.configs import Workload, physics
+    w = configs.workload(2)
+    if a.size != 512:
+        from lbm_bench_gpu_amd import zsc
+        dims = (a.size,) * 3
+        w = Workload(f"zsc-{a.size}^3", dims, physics(*dims),
+                     zsc.generate_bubbles(2, max(1, a.size ** 3 // 1048576), 8, 16, 4.0, 4.0, dims),
+                     a.steps, ("f32",))
+    params = w.params()
+    print(f"# workload {w.name}: fp32 {a.steps} steps", file=sys.stderr, flush=True)
+    main_r = time_precision("f32", w, params, a.steps, a.warmup, a.ty, a.zchunk)
+    extra = {}
+    for v in [s for s in a.variants.split(",") if s]:
+        print(f"# {v} storage", file=sys.stderr, flush=True)
+        extra[v] = time_precision(v, w, params, a.steps, a.warmup, a.ty, a.zchunk)
+    cpu = cpu_baseline(a.cpu_seconds) if a.cpu_seconds > 0 else None
+    out = {
+        "metric": "MLUPS + %HBM roofline, FP32 vs bf16-store (err vs FP64); adjoint MLUPS",
+        "value": round(main_r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(main_r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "cells": w.cells, "bubbles": int(len(w.bubbles)),
+                   "precision": "fp32 storage, fp32 compute", "gravity": "on"},
+        "roofline": main_r["roofline"],
+        "kernel_ms": round(main_r["kernel_ms"], 4),
+        "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
+                         "speedup_vs_fp32": round(v["mlups"] / main_r["mlups"], 3),
+                         "roofline": v["roofline"]} for k, v in extra.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * zsc.h -- C-ABI of the MI355X-native Zheng-Shu-Chew (ZSC) bubbly-flow
+ * lattice-Boltzmann step (libzsc.so).
+ *
+ * The reference (carlosrosales/lbm_bench_gpu) has no FFI: its hot path is the
+ * set of __global__ kernels that main() launches in a fixed order
+ * (src/main.cu:496-590).  Each of those kernels has an entry point here with
+ * the same name (zsc_ prefix), the same argument meaning and the same
+ * structure-of-arrays population layout, so a host driver can keep its call
+ * sequence and swap the launches for these calls.  The differences are:
+ *   - no neighbour-table arguments (nb_east/west/north/south): the periodic
+ *     wrap is done inside the kernels;
+ *   - no ghost planes: a box of nx*ny*nz real cells, N = nx*ny*nz, index
+ *     idx = x + nx*(y + ny*z) (x fastest, src/gridid.c:32-35);
+ *   - populations are passed as arrays of 7 (f) or 19 (g) device pointers,
+ *     each to 2*N elements: the A half [0,N) is post-stream, the B half
+ *     [N,2N) post-collision, exactly the reference's dcol convention
+ *     (src/main.cu:225-251,315);
+ *   - element type is double for ZSC_F64 contexts and float for ZSC_F32;
+ *   - every call returns a status (ZSC_OK or a negative code) and never
+ *     aborts; zsc_last_error() describes the last failure of this thread.
+ *
+ * Calls enqueue asynchronously on the context's HIP stream; zsc_sync() waits.
+ * A context is not re-entrant: use it from one host thread at a time.
+ *
+ * Beyond the drop-in kernels the library owns a fast path: a context-held
+ * lattice pair advanced by zsc_step(), one fused HIP kernel per time step,
+ * with FP64, FP32, or FP32-compute/16-bit-storage precisions.
+ */
+#ifndef ZSC_H
+#define ZSC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ZSC_OK 0
+#define ZSC_ERR_ARG (-1)       /* invalid argument or size */
+#define ZSC_ERR_IO (-2)        /* missing or malformed input file */
+#define ZSC_ERR_HIP (-3)       /* HIP runtime error */
+#define ZSC_ERR_ALLOC (-4)     /* device or host allocation failed */
+#define ZSC_ERR_STATE (-5)     /* call out of order (e.g. step before init) */
+#define ZSC_ERR_DIVERGED (-6)  /* NaN/Inf found in a conserved sum */
+#define ZSC_ERR_PRECISION (-7) /* call not available at this precision */
+
+/* ---- precisions -------------------------------------------------------- */
+typedef enum zsc_precision {
+    ZSC_F64 = 0,  /* FP64 storage and compute (parity mode) */
+    ZSC_F32 = 1,  /* FP32 storage and compute */
+    ZSC_BF16 = 2, /* FP32 compute, populations stored as bf16 deviations */
+    ZSC_FP16 = 3  /* FP32 compute, populations stored as fp16 deviations */
+} zsc_precision;
+
+/* ---- inputs: Input/properties.in and Input/discrete.in ----------------- */
+/* Fields of properties.in in file order (src/paramread.c:40-59), plus two
+ * optional overrides that may follow as "key value" pairs at the end of the
+ * file ("phiStar 1.0", "gravity 1e-6"). */
+typedef struct zsc_inputs {
+    int max_step, relax_step, save, stat;
+    int xmax, ymax, zmax;
+    int block_x, block_y; /* accepted for compatibility; unused */
+    double rho_l, rho_h;
+    double tau_rho, tau_phi;
+    double width, sigma, gamma;
+    double eo;
+    double phi_star; /* > 0 overrides 0.5*(rho_h - rho_l) */
+    double gravity;  /* >= 0 overrides 0.25*Eo*sigma/((rho_h-rho_l)*R0^2) */
+} zsc_inputs;
+
+/* Derived scalars (src/paramread.c:141-170), the set the reference uploads
+ * with 23 cudaMemcpyToSymbol calls plus grav_d (src/main.cu:315-342,479).
+ * Fields may be edited before zsc_set_params, e.g. to treat kappa and
+ * alpha (= beta) as independent parameters. */
+typedef struct zsc_params {
+    double phi_star, phi_star_sq, rho_mean; /* rho_mean = 0.5*(rho_h+rho_l) */
+    double alpha, alpha4, kappa, gamma;
+    double eta, eta2, inv_eta2;
+    double inv_tau_phi, inv_tau_phi_one;
+    double inv_tau_rho_one;
+    double k0, k1, k2, kc0, kc1, kc2;
+    double gravity, grav; /* grav = 2*phi_star*gravity (src/paramread.c:162) */
+    double width, sigma;
+} zsc_params;
+
+/* Reads properties.in (src/paramread.c:38-61).  Returns ZSC_ERR_IO when the
+ * file is missing or a token is malformed, ZSC_ERR_ARG for values the
+ * reference rejects (src/main.cu:167-173: xmax, ymax not multiples of 32 are
+ * accepted here, but non-positive sizes or tau <= 1/2 are not). */
+int zsc_read_inputs(const char *properties_path, zsc_inputs *out);
+
+/* Reads discrete.in (src/paramread.c:64-73) without the reference's
+ * MAX_BUBBLES=20 cap: bubbles[4*i..] = xc, yc, zc, R for i < *n_out.
+ * With bubbles == NULL only the count is returned. */
+int zsc_read_bubbles(const char *discrete_path, double *bubbles, int capacity,
+                     int *n_out);
+
+/* Writes a discrete.in in the reference's format. */
+int zsc_write_bubbles(const char *discrete_path, const double *bubbles, int n);
+
+/* Parameter derivation of src/paramread.c:141-170 in FP64.  r0 is the radius
+ * of the first bubble (src/paramread.c:161). */
+int zsc_derive_params(const zsc_inputs *in, double r0, zsc_params *out);
+
+/* Seeded bubble generator (no reference counterpart; the reference reads
+ * bubbles from discrete.in).  SplitMix64 stream from `seed`; radius uniform in
+ * [r_min, r_max]; centre uniform so that the sphere of radius R + width + 1
+ * stays inside the box (x, y 0-based, z 1-based as src/init.c:51-53);
+ * rejection until every pair of surfaces is at least `gap` apart.
+ * Returns ZSC_ERR_ARG when max_tries draws do not place n bubbles. */
+int zsc_generate_bubbles(uint64_t seed, int n, double r_min, double r_max,
+                         double gap, double width, int nx, int ny, int nz,
+                         int max_tries, double *bubbles);
+
+/* Host initial order parameter of src/init.c:39-59 in FP64: phi = -phi_star,
+ * then for each bubble in order, phi = phi_star*tanh(2(R_c - r)/W) where
+ * r <= R_c + W (last bubble wins, z 1-based). */
+int zsc_init_phi_host(int nx, int ny, int nz, const double *bubbles, int n,
+                      double phi_star, double width, double *phi);
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct zsc_ctx zsc_ctx;
+
+/* Creates a context for an nx*ny*nz periodic box on the current HIP device.
+ * stream: a hipStream_t, or NULL for the context's own stream.
+ * Replaces the allocations and dimension symbols of src/main.cu:180-251,
+ * 315-320 (the context allocates only its own fast-path lattices, lazily). */
+int zsc_create(int nx, int ny, int nz, int precision, void *stream,
+               zsc_ctx **out);
+int zsc_destroy(zsc_ctx *ctx);
+
+/* Replaces the cudaMemcpyToSymbol calls of src/main.cu:322-342 and grav_d. */
+int zsc_set_params(zsc_ctx *ctx, const zsc_params *p);
+
+/* Gravity switch of the relaxation (off, src/main.cu:361-362) and evolution
+ * (on, src/main.cu:479) phases. */
+int zsc_set_gravity(zsc_ctx *ctx, int on);
+
+int zsc_sync(zsc_ctx *ctx);
+const char *zsc_last_error(void);
+
+/* ---- drop-in kernels (reference names, no nb tables, wrap in kernel) --- */
+/* Each pointer of f[7]/g[19] addresses 2*N elements (A | B).  phi, rho, ux,
+ * uy, uz address N elements.  All pointers are device pointers. */
+
+/* src/init_f.cu:27-80: f_A = equilibrium at u = 0 from phi. */
+int zsc_init_f(zsc_ctx *ctx, const void *phi, void *const f[7]);
+/* src/init_g.cu:27-95: g_A = equilibrium at u = 0 from phi, rho. */
+int zsc_init_g(zsc_ctx *ctx, const void *phi, const void *rho,
+               void *const g[19]);
+/* src/update_phi.cu:28-46: phi = sum_i f_A,i. */
+int zsc_update_phi(zsc_ctx *ctx, void *phi, void *const f[7]);
+/* src/update_rho.cu:28-53: rho = sum_i g_A,i. */
+int zsc_update_rho(zsc_ctx *ctx, void *rho, void *const g[19]);
+/* src/update_velocity.cu:28-63: u = sum_i c_i g_A,i / rho (g[0] unused). */
+int zsc_update_velocity(zsc_ctx *ctx, const void *rho, void *ux, void *uy,
+                        void *uz, void *const g[19]);
+/* src/collision_f.cu:27-118: D3Q7 BGK + forcing, A -> B. */
+int zsc_collision_f(zsc_ctx *ctx, const void *phi, const void *rho,
+                    const void *ux, const void *uy, const void *uz,
+                    void *const f[7]);
+/* src/collision_g.cu:28-198: D3Q19 BGK + forcing, A -> B. */
+int zsc_collision_g(zsc_ctx *ctx, const void *phi, const void *rho,
+                    const void *ux, const void *uy, const void *uz,
+                    void *const g[19]);
+/* src/stream_f.cu:28-63 with the halo of src/pack_mpi_f.cu/unpack_mpi_f.cu
+ * and src/pack_mpi.cu/unpack_mpi.cu folded into the wrap: B -> A with the
+ * ZSC smoothing f_A,i(x) = eta f_B,i(x - c_i) + (1 - eta) f_B,i(x). */
+int zsc_stream_f(zsc_ctx *ctx, void *const f[7]);
+/* src/stream_g.cu:28-80 (+ halo as above): g_A,i(x) = g_B,i(x - c_i). */
+int zsc_stream_g(zsc_ctx *ctx, void *const g[19]);
+
+/* ---- fast path: context-owned lattices, one fused kernel per step ------ */
+/* Initial state from a host FP64 order-parameter array (N values) and an
+ * optional host density array (NULL: rho = rho_mean everywhere, as
+ * src/init.c:45).  Runs the init_f/init_g equilibria on the device. */
+int zsc_state_init(zsc_ctx *ctx, const double *phi_host,
+                   const double *rho_host);
+/* Same, with phi computed on the device from bubbles (src/init.c:39-59
+ * semantics, FP64 tanh), for boxes too large for a host array. */
+int zsc_state_init_bubbles(zsc_ctx *ctx, const double *bubbles, int n);
+/* Advances the state by nsteps reference time steps (update -> collide ->
+ * stream each, src/main.cu:496-590) with the current gravity switch. */
+int zsc_step(zsc_ctx *ctx, int nsteps);
+/* Same, and returns the device time of the launches between two HIP events
+ * on the context stream (synchronises). */
+int zsc_step_timed(zsc_ctx *ctx, int nsteps, double *ms);
+/* Post-stream (A) populations as FP64 host arrays, pop-major: f[i*N + idx],
+ * g[i*N + idx].  Storage precision is decoded. */
+int zsc_state_get(zsc_ctx *ctx, double *f_host, double *g_host);
+/* Replaces the state by the given A populations (FP64 host, pop-major). */
+int zsc_state_set(zsc_ctx *ctx, const double *f_host, const double *g_host);
+/* Moments of the A state (phi, rho, u as src/update_*.cu) to FP64 host
+ * arrays of N values; any pointer may be NULL. */
+int zsc_state_moments(zsc_ctx *ctx, double *phi, double *rho, double *ux,
+                      double *uy, double *uz);
+/* Copies the A state into / from reference-style device arrays (A half of
+ * each 2N array; F64 and F32 contexts only). */
+int zsc_state_export(zsc_ctx *ctx, void *const f[7], void *const g[19]);
+int zsc_state_import(zsc_ctx *ctx, void *const f[7], void *const g[19]);
+
+/* Diagnostics of src/stats.c:39-91 and src/logsave.c:63-167, reduced on the
+ * device in FP64 (wavefront shuffles, then a fixed-order block pass). */
+typedef struct zsc_stats {
+    double sum_phi, sum_rho;       /* all cells */
+    double bubble_volume;          /* phi >= 0 cells, x,y in [1,n-2] */
+    double ux_sum, uy_sum, uz_sum; /* over those bubble cells */
+    double umax_sq;                /* max |u|^2 over x,y in [1,n-2] */
+    double p_in_sum, p_out_sum;    /* src/logsave.c:135-143 pressure sums */
+    double bubble_volume_full;     /* phi >= 0 cells, whole box */
+} zsc_stats;
+/* Reduces the moments of the A state.  Returns ZSC_ERR_DIVERGED when a sum
+ * is NaN or Inf. */
+int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
+
+/* Tile shape of the fused kernel: ty wavefront rows of 64 cells per block
+ * (1, 2, 4 or 8) and zchunk planes marched per block (0: the whole box).
+ * No reference counterpart; a tuning knob of this implementation. */
+int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk);
+
+/* Bytes per lattice update the fused step must move (26 populations read
+ * and written once at the storage precision). */
+double zsc_bytes_per_cell(int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZSC_H */

@@ -1,0 +1,613 @@
+// zsc_api.cpp -- the C-ABI of libzsc: contexts, the reference-named drop-in
+// kernels and the fused fast path.  Status codes instead of aborts.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <utility>
+
+#include "../../include/zsc.h"
+#include "zsc_err.h"
+#include "zsc_launch.h"
+
+namespace zsc {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+template <class T>
+struct Tag {
+    using type = T;
+};
+
+// Calls f(Tag<S>{}) for the storage type of a precision.
+template <class F>
+static int dispatch(int prec, F &&f)
+{
+    switch (prec) {
+    case ZSC_F64: return f(Tag<double>{});
+    case ZSC_F32: return f(Tag<float>{});
+    case ZSC_BF16: return f(Tag<bf16s>{});
+    case ZSC_FP16: return f(Tag<f16s>{});
+    default: return fail(ZSC_ERR_ARG, "unknown precision %d", prec);
+    }
+}
+
+static int hip_fail(hipError_t e, const char *what)
+{
+    return fail(ZSC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define ZSC_HIP(call, what)                           \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+}  // namespace zsc
+
+using namespace zsc;
+
+enum { STATE_NONE = 0, STATE_A = 1, STATE_B = 2 };
+
+struct zsc_ctx {
+    Geo geo{};
+    int prec = ZSC_F64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    zsc_params params{};
+    bool have_params = false;
+    int grav_on = 1;
+    // fast-path lattices; lat[cur] holds the state
+    void *lat[2] = {nullptr, nullptr};
+    size_t lat_bytes = 0;
+    int cur = 0;
+    int state = STATE_NONE;
+    // phi, rho, ux, uy, uz at compute precision
+    void *work = nullptr;
+    size_t rsize = 8;
+    double *dbuf = nullptr;  // n doubles of staging for host transfers
+    double *stats_buf = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    FusedCfg cfg{4, 64};
+
+    void *w(int i) const { return (char *)work + (size_t)i * align256((size_t)geo.n * rsize); }
+};
+
+static int need_ctx(zsc_ctx *c)
+{
+    if (!c) return fail(ZSC_ERR_ARG, "null context");
+    return ZSC_OK;
+}
+
+static int set_device(zsc_ctx *c)
+{
+    ZSC_HIP(hipSetDevice(c->device), "hipSetDevice");
+    return ZSC_OK;
+}
+
+extern "C" const char *zsc_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, zsc_ctx **out)
+{
+    if (!out) return fail(ZSC_ERR_ARG, "zsc_create: null output");
+    *out = nullptr;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(ZSC_ERR_ARG, "zsc_create: sizes must be positive");
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (n >= ((int64_t)1 << 31)) return fail(ZSC_ERR_ARG, "zsc_create: more than 2^31-1 cells");
+    if (precision < ZSC_F64 || precision > ZSC_FP16)
+        return fail(ZSC_ERR_ARG, "zsc_create: unknown precision %d", precision);
+    zsc_ctx *c = new zsc_ctx;
+    c->geo = {nx, ny, nz, n};
+    c->prec = precision;
+    c->rsize = precision == ZSC_F64 ? 8 : 4;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) {
+        if (stream) {
+            c->stream = (hipStream_t)stream;
+        } else {
+            e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+            c->own_stream = e == hipSuccess;
+        }
+    }
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        zsc_destroy(c);
+        return hip_fail(e, "zsc_create");
+    }
+    *out = c;
+    return ZSC_OK;
+}
+
+extern "C" int zsc_destroy(zsc_ctx *c)
+{
+    if (!c) return ZSC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : {c->lat[0], c->lat[1], c->work, (void *)c->dbuf, (void *)c->stats_buf})
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ZSC_OK;
+}
+
+extern "C" int zsc_set_params(zsc_ctx *c, const zsc_params *p)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!p) return fail(ZSC_ERR_ARG, "zsc_set_params: null params");
+    if (!(p->eta > 0.0 && p->eta < 1.0) || !isfinite(p->inv_eta2))
+        return fail(ZSC_ERR_ARG, "zsc_set_params: eta must lie in (0,1)");
+    c->params = *p;
+    c->have_params = true;
+    return ZSC_OK;
+}
+
+extern "C" int zsc_set_gravity(zsc_ctx *c, int on)
+{
+    if (int r = need_ctx(c)) return r;
+    c->grav_on = on ? 1 : 0;
+    return ZSC_OK;
+}
+
+extern "C" int zsc_set_fused_config(zsc_ctx *c, int ty, int zchunk)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || zchunk < 0)
+        return fail(ZSC_ERR_ARG, "zsc_set_fused_config: ty in {1,2,4,8}, zchunk >= 0");
+    c->cfg = {ty, zchunk};
+    return ZSC_OK;
+}
+
+extern "C" int zsc_sync(zsc_ctx *c)
+{
+    if (int r = need_ctx(c)) return r;
+    ZSC_HIP(hipStreamSynchronize(c->stream), "zsc_sync");
+    return ZSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// drop-in kernels
+// ---------------------------------------------------------------------------
+template <class F>
+static int dropin(zsc_ctx *c, bool need_params, const char *name, F &&f)
+{
+    if (int r = need_ctx(c)) return r;
+    if (need_params && !c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
+    if (int r = set_device(c)) return r;
+    hipError_t e;
+    if (c->prec == ZSC_F64)
+        e = f(Tag<double>{});
+    else if (c->prec == ZSC_F32)
+        e = f(Tag<float>{});
+    else
+        return fail(ZSC_ERR_PRECISION, "%s: drop-in kernels need an F64 or F32 context", name);
+    if (e != hipSuccess) return hip_fail(e, name);
+    return ZSC_OK;
+}
+
+template <class T, int Q>
+static bool all_set(void *const *p)
+{
+    if (!p) return false;
+    for (int i = 0; i < Q; ++i)
+        if (!p[i]) return false;
+    return true;
+}
+
+#define ZSC_T typename decltype(tag)::type
+
+extern "C" int zsc_init_f(zsc_ctx *c, const void *phi, void *const f[7])
+{
+    if (!phi || !all_set<void, 7>(f)) return fail(ZSC_ERR_ARG, "zsc_init_f: null pointer");
+    return dropin(c, true, "zsc_init_f", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::init_f(c->geo, c->params, c->stream, (const T *)phi, (T *const *)f);
+    });
+}
+
+extern "C" int zsc_init_g(zsc_ctx *c, const void *phi, const void *rho, void *const g[19])
+{
+    if (!phi || !rho || !all_set<void, 19>(g)) return fail(ZSC_ERR_ARG, "zsc_init_g: null pointer");
+    return dropin(c, true, "zsc_init_g", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::init_g(c->geo, c->params, c->stream, (const T *)phi, (const T *)rho,
+                                 (T *const *)g);
+    });
+}
+
+extern "C" int zsc_update_phi(zsc_ctx *c, void *phi, void *const f[7])
+{
+    if (!phi || !all_set<void, 7>(f)) return fail(ZSC_ERR_ARG, "zsc_update_phi: null pointer");
+    return dropin(c, false, "zsc_update_phi", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::update_phi(c->geo, c->stream, (T *)phi, (T *const *)f);
+    });
+}
+
+extern "C" int zsc_update_rho(zsc_ctx *c, void *rho, void *const g[19])
+{
+    if (!rho || !all_set<void, 19>(g)) return fail(ZSC_ERR_ARG, "zsc_update_rho: null pointer");
+    return dropin(c, false, "zsc_update_rho", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::update_rho(c->geo, c->stream, (T *)rho, (T *const *)g);
+    });
+}
+
+extern "C" int zsc_update_velocity(zsc_ctx *c, const void *rho, void *ux, void *uy, void *uz,
+                                   void *const g[19])
+{
+    if (!rho || !ux || !uy || !uz || !g) return fail(ZSC_ERR_ARG, "zsc_update_velocity: null pointer");
+    for (int i = 1; i < 19; ++i)
+        if (!g[i]) return fail(ZSC_ERR_ARG, "zsc_update_velocity: g[%d] is null", i);
+    return dropin(c, false, "zsc_update_velocity", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::update_velocity(c->geo, c->stream, (const T *)rho, (T *)ux, (T *)uy,
+                                          (T *)uz, (T *const *)g);
+    });
+}
+
+extern "C" int zsc_collision_f(zsc_ctx *c, const void *phi, const void *rho, const void *ux,
+                               const void *uy, const void *uz, void *const f[7])
+{
+    if (!phi || !rho || !ux || !uy || !uz || !all_set<void, 7>(f))
+        return fail(ZSC_ERR_ARG, "zsc_collision_f: null pointer");
+    return dropin(c, true, "zsc_collision_f", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::collision_f(c->geo, c->params, c->grav_on, c->stream, (const T *)phi,
+                                      (const T *)rho, (const T *)ux, (const T *)uy,
+                                      (const T *)uz, (T *const *)f);
+    });
+}
+
+extern "C" int zsc_collision_g(zsc_ctx *c, const void *phi, const void *rho, const void *ux,
+                               const void *uy, const void *uz, void *const g[19])
+{
+    if (!phi || !rho || !ux || !uy || !uz || !all_set<void, 19>(g))
+        return fail(ZSC_ERR_ARG, "zsc_collision_g: null pointer");
+    return dropin(c, true, "zsc_collision_g", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::collision_g(c->geo, c->params, c->grav_on, c->stream, (const T *)phi,
+                                      (const T *)rho, (const T *)ux, (const T *)uy,
+                                      (const T *)uz, (T *const *)g);
+    });
+}
+
+extern "C" int zsc_stream_f(zsc_ctx *c, void *const f[7])
+{
+    if (!all_set<void, 7>(f)) return fail(ZSC_ERR_ARG, "zsc_stream_f: null pointer");
+    return dropin(c, true, "zsc_stream_f", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::stream_f(c->geo, c->params, c->stream, (T *const *)f);
+    });
+}
+
+extern "C" int zsc_stream_g(zsc_ctx *c, void *const g[19])
+{
+    if (!all_set<void, 19>(g)) return fail(ZSC_ERR_ARG, "zsc_stream_g: null pointer");
+    return dropin(c, false, "zsc_stream_g", [&](auto tag) {
+        using T = ZSC_T;
+        return DropIn<T>::stream_g(c->geo, c->stream, (T *const *)g);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// fast path
+// ---------------------------------------------------------------------------
+static int ensure_alloc(zsc_ctx *c)
+{
+    if (c->lat[0]) return ZSC_OK;
+    if (int r = set_device(c)) return r;
+    size_t lb = 0;
+    dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        lb = LatLayout<S>::bytes(c->geo.n);
+        return ZSC_OK;
+    });
+    const size_t wb = 5 * align256((size_t)c->geo.n * c->rsize);
+    void *a = nullptr, *b = nullptr, *w = nullptr, *s = nullptr;
+    hipError_t e = hipMalloc(&a, lb);
+    if (e == hipSuccess) e = hipMalloc(&b, lb);
+    if (e == hipSuccess) e = hipMalloc(&w, wb);
+    if (e == hipSuccess) e = hipMalloc(&s, (kStatsBlocks + 1) * 10 * sizeof(double));
+    if (e != hipSuccess) {
+        for (void *p : {a, b, w, s})
+            if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+        return fail(ZSC_ERR_ALLOC, "cannot allocate %.2f GB of lattices and work arrays",
+                    (2.0 * lb + wb) / 1e9);
+    }
+    c->lat[0] = a;
+    c->lat[1] = b;
+    c->lat_bytes = lb;
+    c->work = w;
+    c->stats_buf = (double *)s;
+    return ZSC_OK;
+}
+
+static int ensure_dbuf(zsc_ctx *c)
+{
+    if (c->dbuf) return ZSC_OK;
+    if (hipMalloc((void **)&c->dbuf, (size_t)c->geo.n * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        c->dbuf = nullptr;
+        return fail(ZSC_ERR_ALLOC, "cannot allocate the FP64 staging buffer");
+    }
+    return ZSC_OK;
+}
+
+// Brings the state into the A (post-stream) convention.
+static int materialize_a(zsc_ctx *c)
+{
+    if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "no state: call zsc_state_init first");
+    if (c->state == STATE_A) return ZSC_OK;
+    const int nxt = 1 - c->cur;
+    int r = dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        hipError_t e = Ops<S>::lat_stream(c->geo, c->params, c->stream, c->lat[c->cur], c->lat[nxt]);
+        return e == hipSuccess ? ZSC_OK : hip_fail(e, "stream");
+    });
+    if (r) return r;
+    c->cur = nxt;
+    c->state = STATE_A;
+    return ZSC_OK;
+}
+
+static int init_from_work(zsc_ctx *c, bool have_rho)
+{
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        hipError_t e = Ops<S>::lat_init(c->geo, c->params, c->stream, c->w(0),
+                                        have_rho ? c->w(1) : nullptr, c->lat[c->cur]);
+        if (e != hipSuccess) return hip_fail(e, "state init");
+        c->state = STATE_A;
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_state_init(zsc_ctx *c, const double *phi_host, const double *rho_host)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!phi_host) return fail(ZSC_ERR_ARG, "zsc_state_init: null phi");
+    if (!c->have_params) return fail(ZSC_ERR_STATE, "zsc_state_init: zsc_set_params first");
+    if (int r = ensure_alloc(c)) return r;
+    if (int r = ensure_dbuf(c)) return r;
+    const size_t bytes = (size_t)c->geo.n * sizeof(double);
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        ZSC_HIP(hipMemcpyAsync(c->dbuf, phi_host, bytes, hipMemcpyHostToDevice, c->stream), "phi upload");
+        ZSC_HIP(Ops<S>::from_f64(c->stream, c->geo.n, c->dbuf, c->w(0)), "phi convert");
+        if (rho_host) {
+            ZSC_HIP(hipStreamSynchronize(c->stream), "sync");
+            ZSC_HIP(hipMemcpyAsync(c->dbuf, rho_host, bytes, hipMemcpyHostToDevice, c->stream), "rho upload");
+            ZSC_HIP(Ops<S>::from_f64(c->stream, c->geo.n, c->dbuf, c->w(1)), "rho convert");
+        }
+        int r = init_from_work(c, rho_host != nullptr);
+        if (r) return r;
+        ZSC_HIP(hipStreamSynchronize(c->stream), "state init");
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_state_init_bubbles(zsc_ctx *c, const double *bubbles, int n)
+{
+    if (int r = need_ctx(c)) return r;
+    if (n < 0 || (n > 0 && !bubbles)) return fail(ZSC_ERR_ARG, "zsc_state_init_bubbles: bad bubbles");
+    if (!c->have_params) return fail(ZSC_ERR_STATE, "zsc_state_init_bubbles: zsc_set_params first");
+    if (int r = ensure_alloc(c)) return r;
+    double *db = nullptr;
+    const size_t bb = (size_t)(n > 0 ? n : 1) * 4 * sizeof(double);
+    ZSC_HIP(hipMalloc((void **)&db, bb), "bubble buffer");
+    int r = dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        if (n > 0) ZSC_HIP(hipMemcpyAsync(db, bubbles, bb, hipMemcpyHostToDevice, c->stream), "bubbles");
+        ZSC_HIP(Ops<S>::init_phi_bubbles(c->geo, c->stream, db, n, c->params.phi_star,
+                                         c->params.width, c->w(0)),
+                "init phi");
+        int rr = init_from_work(c, false);
+        if (rr) return rr;
+        ZSC_HIP(hipStreamSynchronize(c->stream), "state init");
+        return ZSC_OK;
+    });
+    (void)hipFree(db);
+    return r;
+}
+
+static int step_impl(zsc_ctx *c, int nsteps)
+{
+    if (int r = need_ctx(c)) return r;
+    if (nsteps < 0) return fail(ZSC_ERR_ARG, "zsc_step: negative step count");
+    if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "zsc_step: no state");
+    if (nsteps == 0) return ZSC_OK;
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        int left = nsteps;
+        if (c->state == STATE_A) {
+            // first update + collision from the A lattice, through the phi array
+            const int nxt = 1 - c->cur;
+            ZSC_HIP(Ops<S>::lat_phi(c->geo, c->stream, c->lat[c->cur], c->w(0)), "phi");
+            ZSC_HIP(Ops<S>::lat_collide(c->geo, c->params, c->grav_on, c->stream, c->lat[c->cur],
+                                        c->w(0), c->lat[nxt]),
+                    "collide");
+            c->cur = nxt;
+            c->state = STATE_B;
+            --left;
+        }
+        // each fused launch: stream of step k, update + collision of step k+1
+        for (; left > 0; --left) {
+            const int nxt = 1 - c->cur;
+            ZSC_HIP(Ops<S>::lat_fused(c->geo, c->params, c->grav_on, c->stream, c->cfg,
+                                      c->lat[c->cur], c->lat[nxt]),
+                    "fused step");
+            c->cur = nxt;
+        }
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_step(zsc_ctx *c, int nsteps) { return step_impl(c, nsteps); }
+
+extern "C" int zsc_step_timed(zsc_ctx *c, int nsteps, double *ms)
+{
+    if (int r = need_ctx(c)) return r;
+    ZSC_HIP(hipEventRecord(c->ev0, c->stream), "event");
+    if (int r = step_impl(c, nsteps)) return r;
+    ZSC_HIP(hipEventRecord(c->ev1, c->stream), "event");
+    ZSC_HIP(hipEventSynchronize(c->ev1), "event sync");
+    float t = 0.f;
+    ZSC_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1), "elapsed");
+    if (ms) *ms = t;
+    return ZSC_OK;
+}
+
+extern "C" int zsc_state_get(zsc_ctx *c, double *f_host, double *g_host)
+{
+    if (int r = need_ctx(c)) return r;
+    if (int r = materialize_a(c)) return r;
+    if (int r = ensure_dbuf(c)) return r;
+    const size_t bytes = (size_t)c->geo.n * sizeof(double);
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        for (int q = 0; q < 26; ++q) {
+            double *dst = q < 7 ? (f_host ? f_host + (size_t)q * c->geo.n : nullptr)
+                                : (g_host ? g_host + (size_t)(q - 7) * c->geo.n : nullptr);
+            if (!dst) continue;
+            ZSC_HIP(Ops<S>::pop_get(c->geo, c->params, c->stream, c->lat[c->cur], q, c->dbuf), "decode");
+            ZSC_HIP(hipMemcpyAsync(dst, c->dbuf, bytes, hipMemcpyDeviceToHost, c->stream), "download");
+            ZSC_HIP(hipStreamSynchronize(c->stream), "download");
+        }
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_state_set(zsc_ctx *c, const double *f_host, const double *g_host)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!f_host || !g_host) return fail(ZSC_ERR_ARG, "zsc_state_set: null arrays");
+    if (!c->have_params) return fail(ZSC_ERR_STATE, "zsc_state_set: zsc_set_params first");
+    if (int r = ensure_alloc(c)) return r;
+    if (int r = ensure_dbuf(c)) return r;
+    const size_t bytes = (size_t)c->geo.n * sizeof(double);
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        for (int q = 0; q < 26; ++q) {
+            const double *src = q < 7 ? f_host + (size_t)q * c->geo.n : g_host + (size_t)(q - 7) * c->geo.n;
+            ZSC_HIP(hipMemcpyAsync(c->dbuf, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+            ZSC_HIP(Ops<S>::pop_set(c->geo, c->params, c->stream, c->lat[c->cur], q, c->dbuf), "encode");
+            ZSC_HIP(hipStreamSynchronize(c->stream), "upload");
+        }
+        c->state = STATE_A;
+        return ZSC_OK;
+    });
+}
+
+static int moments_to_work(zsc_ctx *c)
+{
+    if (int r = materialize_a(c)) return r;
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        ZSC_HIP(Ops<S>::lat_moments(c->geo, c->params, c->stream, c->lat[c->cur], c->w(0), c->w(1),
+                                    c->w(2), c->w(3), c->w(4)),
+                "moments");
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_state_moments(zsc_ctx *c, double *phi, double *rho, double *ux, double *uy,
+                                 double *uz)
+{
+    if (int r = need_ctx(c)) return r;
+    if (int r = moments_to_work(c)) return r;
+    if (int r = ensure_dbuf(c)) return r;
+    const size_t bytes = (size_t)c->geo.n * sizeof(double);
+    double *outs[5] = {phi, rho, ux, uy, uz};
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        for (int i = 0; i < 5; ++i) {
+            if (!outs[i]) continue;
+            ZSC_HIP(Ops<S>::to_f64(c->stream, c->geo.n, c->w(i), c->dbuf), "convert");
+            ZSC_HIP(hipMemcpyAsync(outs[i], c->dbuf, bytes, hipMemcpyDeviceToHost, c->stream), "download");
+            ZSC_HIP(hipStreamSynchronize(c->stream), "download");
+        }
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_state_stats(zsc_ctx *c, zsc_stats *out)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!out) return fail(ZSC_ERR_ARG, "zsc_state_stats: null output");
+    if (int r = moments_to_work(c)) return r;
+    double h[10];
+    int r = dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        double *partial = c->stats_buf + 10;
+        ZSC_HIP(Ops<S>::stats(c->geo, c->params, c->stream, c->w(0), c->w(1), c->w(2), c->w(3),
+                              c->w(4), partial, c->stats_buf),
+                "stats");
+        ZSC_HIP(hipMemcpyAsync(h, c->stats_buf, sizeof h, hipMemcpyDeviceToHost, c->stream), "stats");
+        ZSC_HIP(hipStreamSynchronize(c->stream), "stats");
+        return ZSC_OK;
+    });
+    if (r) return r;
+    out->sum_phi = h[0];
+    out->sum_rho = h[1];
+    out->bubble_volume = h[2];
+    out->ux_sum = h[3];
+    out->uy_sum = h[4];
+    out->uz_sum = h[5];
+    out->umax_sq = h[6];
+    out->p_in_sum = h[7];
+    out->p_out_sum = h[8];
+    out->bubble_volume_full = h[9];
+    if (!isfinite(h[0]) || !isfinite(h[1]) || !isfinite(h[6]))
+        return fail(ZSC_ERR_DIVERGED, "non-finite conserved sum (sum phi %g, sum rho %g)", h[0], h[1]);
+    return ZSC_OK;
+}
+
+template <bool Export>
+static int xport(zsc_ctx *c, void *const *f, void *const *g, const char *name)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!all_set<void, 7>(f) || !all_set<void, 19>(g)) return fail(ZSC_ERR_ARG, "%s: null pointer", name);
+    if (Export) {
+        if (int r = materialize_a(c)) return r;
+    } else {
+        if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
+        if (int r = ensure_alloc(c)) return r;
+    }
+    return dropin(c, false, name, [&](auto tag) {
+        using T = ZSC_T;
+        if (Export) return DropIn<T>::export_state(c->geo, c->stream, c->lat[c->cur], (T *const *)f, (T *const *)g);
+        hipError_t e = DropIn<T>::import_state(c->geo, c->stream, c->lat[c->cur], (T *const *)f, (T *const *)g);
+        if (e == hipSuccess) c->state = STATE_A;
+        return e;
+    });
+}
+
+extern "C" int zsc_state_export(zsc_ctx *c, void *const f[7], void *const g[19])
+{
+    return xport<true>(c, f, g, "zsc_state_export");
+}
+
+extern "C" int zsc_state_import(zsc_ctx *c, void *const f[7], void *const g[19])
+{
+    return xport<false>(c, f, g, "zsc_state_import");
+}

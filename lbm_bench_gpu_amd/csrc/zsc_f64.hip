@@ -1,0 +1,9 @@
+// zsc_f64.hip -- FP64 instantiation of libzsc's kernels (parity mode).
+// Built with -ffp-contract=off: every multiply and add is rounded on its own,
+// as in an IEEE evaluation of the reference's expressions.
+#include "zsc_kernels.h"
+
+namespace zsc {
+template struct Ops<double>;
+template struct DropIn<double>;
+}  // namespace zsc

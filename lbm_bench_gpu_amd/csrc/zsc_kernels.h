@@ -1,0 +1,968 @@
+// zsc_kernels.h -- HIP kernels of libzsc (gfx950), templated on the
+// population storage type.  Included by exactly one translation unit per
+// precision family, which instantiates Ops<S> / DropIn<T>.
+//
+// Layout: fully periodic nx*ny*nz box, idx = x + nx*(y + ny*z), x fastest
+// (src/gridid.c:32-35), structure-of-arrays populations.  Every kernel wraps
+// neighbour indices itself; this replaces the neighbour tables of
+// src/init.c:62-82 and the z ghost planes + halo exchange of src/main.cu:515-589.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "zsc_cell.h"
+#include "zsc_launch.h"
+#include "zsc_store.h"
+
+namespace zsc {
+
+template <class R>
+inline Coef<R> make_coef(const zsc_params &p, int grav_on)
+{
+    Coef<R> c;
+    c.phi_star_sq = (R)p.phi_star_sq;
+    c.alpha4 = (R)p.alpha4;
+    c.kappa = (R)p.kappa;
+    c.gamma = (R)p.gamma;
+    c.eta = (R)p.eta;
+    c.eta2 = (R)p.eta2;
+    c.inv_eta2 = (R)p.inv_eta2;
+    c.inv_tau_phi = (R)p.inv_tau_phi;
+    c.inv_tau_phi_one = (R)p.inv_tau_phi_one;
+    c.inv_tau_rho_one = (R)p.inv_tau_rho_one;
+    c.k0 = (R)p.k0;
+    c.k1 = (R)p.k1;
+    c.k2 = (R)p.k2;
+    c.kc0 = (R)p.kc0;
+    c.kc1 = (R)p.kc1;
+    c.kc2 = (R)p.kc2;
+    c.grav = grav_on ? (R)p.grav : R(0);
+    c.rho_mean = (R)p.rho_mean;
+    return c;
+}
+
+__device__ inline int wrap_dec(int v, int n) { return v == 0 ? n - 1 : v - 1; }
+__device__ inline int wrap_inc(int v, int n) { return v == n - 1 ? 0 : v + 1; }
+
+// Neighbour offsets of one cell: cols[1+dx] + rows[1+dy] + planes[1+dz] is
+// the linear index of (x+dx, y+dy, z+dz), periodic.
+struct Nbr {
+    int64_t cols[3], rows[3], planes[3];
+    __device__ Nbr(int x, int y, int z, int nx, int ny, int nz)
+    {
+        const int64_t nxy = (int64_t)nx * ny;
+        cols[0] = wrap_dec(x, nx);
+        cols[1] = x;
+        cols[2] = wrap_inc(x, nx);
+        rows[0] = (int64_t)wrap_dec(y, ny) * nx;
+        rows[1] = (int64_t)y * nx;
+        rows[2] = (int64_t)wrap_inc(y, ny) * nx;
+        planes[0] = (int64_t)wrap_dec(z, nz) * nxy;
+        planes[1] = (int64_t)z * nxy;
+        planes[2] = (int64_t)wrap_inc(z, nz) * nxy;
+    }
+    // index of x + (dx, dy, dz)
+    __device__ int64_t at(int dx, int dy, int dz) const
+    {
+        return cols[1 + dx] + rows[1 + dy] + planes[1 + dz];
+    }
+};
+
+template <class R>
+__device__ inline void gather19(const R *__restrict__ phi, const Nbr &nb, R *p)
+{
+#pragma unroll
+    for (int d = 0; d < 19; ++d) p[d] = phi[nb.at(cx(d), cy(d), cz(d))];
+}
+
+__device__ inline void cell_xyz(int64_t idx, int nx, int ny, int &x, int &y, int &z)
+{
+    const int64_t t = idx / nx;
+    x = (int)(idx - t * nx);
+    y = (int)(t % ny);
+    z = (int)(t / ny);
+}
+
+#define ZSC_GRID_STRIDE(idx, n) \
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (n); \
+         idx += (int64_t)gridDim.x * blockDim.x)
+
+inline dim3 grid_for(int64_t n, int block = 256)
+{
+    int64_t b = (n + block - 1) / block;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+// ===========================================================================
+// Context-lattice kernels
+// ===========================================================================
+
+template <class S, class R>
+__device__ inline R ldg(const S *p, int64_t i, R rest)
+{
+    return Codec<S>::dec(p[i], rest);
+}
+template <class S, class R>
+__device__ inline void stg(S *p, int64_t i, R v, R rest)
+{
+    p[i] = Codec<S>::enc(v, rest);
+}
+
+template <class R>
+__device__ inline R g_rest(const Coef<R> &c, int i)
+{
+    return g_weight<R>(i) * c.rho_mean;
+}
+
+// Decoded A populations of one cell of a lattice.
+template <class S, class R>
+__device__ inline void load_local(const Lattice<S> &L, int64_t n, int64_t idx, const Coef<R> &c,
+                                  R *f, R *g)
+{
+    f[0] = ldg(L.f0, idx, R(0));
+#pragma unroll
+    for (int i = 1; i < 7; ++i) f[i] = ldg(L.f, (int64_t)(i - 1) * n + idx, R(0));
+#pragma unroll
+    for (int i = 0; i < 19; ++i) g[i] = ldg(L.g, (int64_t)i * n + idx, g_rest(c, i));
+}
+
+template <class S, class R>
+__device__ inline void store_local(const Lattice<S> &L, int64_t n, int64_t idx, const Coef<R> &c,
+                                   const R *f, const R *g)
+{
+    stg(L.f0, idx, f[0], R(0));
+#pragma unroll
+    for (int i = 1; i < 7; ++i) stg(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
+#pragma unroll
+    for (int i = 0; i < 19; ++i) stg(L.g, (int64_t)i * n + idx, g[i], g_rest(c, i));
+}
+
+// Post-stream f of one cell, pulled from a B lattice (src/stream_f.cu:52-59
+// as a pull): f_A,i(y) = eta f_B,i(y - c_i) + eta2 f_B,i(y).
+template <class S, class R>
+__device__ inline void pull_f(const Lattice<S> &B, int64_t n, const Nbr &nb, const Coef<R> &c,
+                              R *f)
+{
+    const int64_t own = nb.at(0, 0, 0);
+    f[0] = ldg(B.f0, own, R(0));
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+        const S *p = B.f + (int64_t)(i - 1) * n;
+        f[i] = c.eta * ldg(p, nb.at(-cx(i), -cy(i), -cz(i)), R(0)) + c.eta2 * ldg(p, own, R(0));
+    }
+}
+
+// Post-stream g of one cell (src/stream_g.cu:56-76 as a pull).
+template <class S, class R>
+__device__ inline void pull_g(const Lattice<S> &B, int64_t n, const Nbr &nb, const Coef<R> &c,
+                              R *g)
+{
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+        g[i] = ldg(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_lat_init(int nx, int ny, int nz, int64_t n, Coef<typename Traits<S>::R> c,
+               const typename Traits<S>::R *__restrict__ phi,
+               const typename Traits<S>::R *__restrict__ rho, Lattice<S> A)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        R p[19], f[7], g[19];
+        gather19(phi, nb, p);
+        init_f_cell(c, p, f);
+        init_g_cell(c, p, rho ? rho[idx] : c.rho_mean, g);
+        store_local(A, n, idx, c, f, g);
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_lat_phi(int64_t n, Coef<typename Traits<S>::R> c, Lattice<S> A,
+              typename Traits<S>::R *__restrict__ phi)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        R f[7];
+        f[0] = ldg(A.f0, idx, R(0));
+#pragma unroll
+        for (int i = 1; i < 7; ++i) f[i] = ldg(A.f, (int64_t)(i - 1) * n + idx, R(0));
+        phi[idx] = sum_f(f);
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_lat_collide(int nx, int ny, int nz, int64_t n, Coef<typename Traits<S>::R> c, Lattice<S> A,
+                  const typename Traits<S>::R *__restrict__ phi, Lattice<S> B)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        R p[19], fa[7], ga[19], fb[7], gb[19];
+        gather19(phi, nb, p);
+        load_local(A, n, idx, c, fa, ga);
+        collide_cell(c, p, fa, ga, fb, gb);
+        store_local(B, n, idx, c, fb, gb);
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_lat_stream(int nx, int ny, int nz, int64_t n, Coef<typename Traits<S>::R> c, Lattice<S> B,
+                 Lattice<S> A)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        R f[7], g[19];
+        pull_f(B, n, nb, c, f);
+        pull_g(B, n, nb, c, g);
+        store_local(A, n, idx, c, f, g);
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_lat_moments(int64_t n, Coef<typename Traits<S>::R> c, Lattice<S> A,
+                  typename Traits<S>::R *phi, typename Traits<S>::R *rho,
+                  typename Traits<S>::R *ux, typename Traits<S>::R *uy,
+                  typename Traits<S>::R *uz)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        R f[7], g[19];
+        load_local(A, n, idx, c, f, g);
+        const R r = sum_g(g);
+        R vx, vy, vz;
+        velocity(g, r, vx, vy, vz);
+        if (phi) phi[idx] = sum_f(f);
+        if (rho) rho[idx] = r;
+        if (ux) ux[idx] = vx;
+        if (uy) uy[idx] = vy;
+        if (uz) uz[idx] = vz;
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_pop_get(int64_t n, const typename Traits<S>::S0 *__restrict__ f0,
+              const S *__restrict__ p, int is_f0, typename Traits<S>::R rest, double *dst)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        dst[idx] = is_f0 ? (double)ldg(f0, idx, R(0)) : (double)ldg(p, idx, rest);
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_pop_set(int64_t n, typename Traits<S>::S0 *__restrict__ f0, S *__restrict__ p, int is_f0,
+              typename Traits<S>::R rest, const double *src)
+{
+    using R = typename Traits<S>::R;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        if (is_f0)
+            stg(f0, idx, (R)src[idx], R(0));
+        else
+            stg(p, idx, (R)src[idx], rest);
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_to_f64(int64_t n, const R *src, double *dst)
+{
+    ZSC_GRID_STRIDE(idx, n) dst[idx] = (double)src[idx];
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_from_f64(int64_t n, const double *src, R *dst)
+{
+    ZSC_GRID_STRIDE(idx, n) dst[idx] = (R)src[idx];
+}
+
+// Initial phi of src/init.c:39-59 in FP64 (last bubble wins, z 1-based).
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_init_phi_bubbles(int nx, int ny, int64_t n, const double *__restrict__ bub, int nb,
+                       double phi_star, double width, R *phi)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        double v = -phi_star;
+        for (int m = 0; m < nb; ++m) {
+            const double rc = bub[4 * m + 3];
+            const double reach = rc + width;
+            const double dx = (double)x - bub[4 * m + 0];
+            if (dx > reach || dx < -reach) continue;
+            const double dy = (double)y - bub[4 * m + 1];
+            if (dy > reach || dy < -reach) continue;
+            const double dz = (double)(z + 1) - bub[4 * m + 2];
+            if (dz > reach || dz < -reach) continue;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r <= reach) v = phi_star * tanh(2.0 * (rc - r) / width);
+        }
+        phi[idx] = (R)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The fused step: Bout = collide(stream(Bin)).
+//
+// A block owns a 64 x TY column tile (one wavefront per row, lanes along x)
+// and marches a chunk of z planes.  For plane k+1 it pulls the 7 f
+// populations of every tile cell and of a 1-cell ring around the tile,
+// sums them to phi and keeps the plane in an LDS ring of 4 planes; the own
+// cell's pulled f stays in registers for the next iteration.  Plane k is then
+// collided with its 19-point phi stencil read from LDS and 19 pulled g, and
+// written once, coalesced, to Bout.  Four LDS slots make one barrier per plane
+// sufficient: a wave writing slot (k+2)&3 can only run ahead of a wave that
+// still reads slots k-1..k+1 by one iteration.
+// ---------------------------------------------------------------------------
+template <class S, int TY>
+__global__ void __launch_bounds__(64 * TY)
+    k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk,
+            Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out)
+{
+    using R = typename Traits<S>::R;
+    __shared__ R sp[4][TY + 2][66];
+
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of tiles so that neighbouring rows share its L2.
+    const int b = blockIdx.x;
+    int l = b;
+    if ((nblk & 7) == 0) l = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int bx = l % nbx;
+    const int by = (l / nbx) % nby;
+    const int bz = l / (nbx * nby);
+    const int x0 = bx * 64, y0 = by * TY, z0 = bz * zchunk;
+    const int z1 = min(nz, z0 + zchunk);
+    const int tw = min(64, nx - x0), th = min(TY, ny - y0);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const bool own = tx < tw && ty < th;
+    const int x = x0 + min(tx, tw - 1), y = y0 + min(ty, th - 1);
+    const int nh = 2 * tw + 2 * (th + 2);  // ring cells per plane
+    const int tid = ty * 64 + tx;
+
+    // phi of plane zz (tile + ring) into LDS slot; own pulled f into fown.
+    auto plane_phi = [&](int zz, R *fown) {
+        const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        const int slot = (zz + 4) & 3;
+        if (own) {
+            Nbr nb(x, y, zc, nx, ny, nz);
+            pull_f(in, n, nb, c, fown);
+            sp[slot][ty + 1][tx + 1] = sum_f(fown);
+        }
+        for (int h = tid; h < nh; h += 64 * TY) {
+            int hx, hy, r, col;
+            if (h < tw) {  // row below the tile
+                col = h + 1;
+                r = 0;
+                hx = x0 + h;
+                hy = y0 - 1;
+            } else if (h < 2 * tw) {  // row above
+                col = h - tw + 1;
+                r = th + 1;
+                hx = x0 + h - tw;
+                hy = y0 + th;
+            } else if (h < 2 * tw + th + 2) {  // column left, corners included
+                col = 0;
+                r = h - 2 * tw;
+                hx = x0 - 1;
+                hy = y0 - 1 + r;
+            } else {  // column right
+                col = tw + 1;
+                r = h - 2 * tw - th - 2;
+                hx = x0 + tw;
+                hy = y0 - 1 + r;
+            }
+            hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
+            hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+            Nbr nb(hx, hy, zc, nx, ny, nz);
+            R f[7];
+            pull_f(in, n, nb, c, f);
+            sp[slot][r][col] = sum_f(f);
+        }
+    };
+
+    R fcur[7], fnext[7];
+    plane_phi(z0 - 1, fnext);
+    plane_phi(z0, fcur);
+    for (int k = z0; k < z1; ++k) {
+        R ga[19];
+        Nbr nb(x, y, k, nx, ny, nz);
+        if (own) pull_g(in, n, nb, c, ga);
+        plane_phi(k + 1, fnext);
+        __syncthreads();
+        if (own) {
+            const int sm = (k + 3) & 3, s0 = k & 3, spl = (k + 1) & 3;
+            const int r = ty + 1, q = tx + 1;
+            R p[19];
+            p[0] = sp[s0][r][q];
+            p[1] = sp[s0][r][q + 1];
+            p[2] = sp[s0][r][q - 1];
+            p[3] = sp[s0][r + 1][q];
+            p[4] = sp[s0][r - 1][q];
+            p[5] = sp[spl][r][q];
+            p[6] = sp[sm][r][q];
+            p[7] = sp[s0][r + 1][q + 1];
+            p[8] = sp[s0][r - 1][q - 1];
+            p[9] = sp[s0][r - 1][q + 1];
+            p[10] = sp[s0][r + 1][q - 1];
+            p[11] = sp[spl][r][q + 1];
+            p[12] = sp[sm][r][q - 1];
+            p[13] = sp[sm][r][q + 1];
+            p[14] = sp[spl][r][q - 1];
+            p[15] = sp[spl][r + 1][q];
+            p[16] = sp[sm][r - 1][q];
+            p[17] = sp[sm][r + 1][q];
+            p[18] = sp[spl][r - 1][q];
+            R fb[7], gb[19];
+            collide_cell(c, p, fcur, ga, fb, gb);
+            store_local(out, n, nb.at(0, 0, 0), c, fb, gb);
+        }
+#pragma unroll
+        for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
+    }
+}
+
+// FP64 reductions for zsc_stats (src/stats.c:39-55, src/logsave.c:63-143).
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_stats(int nx, int ny, int nz, int64_t n, double alpha, double kappa, double phi_star_sq,
+            const R *__restrict__ phi, const R *__restrict__ rho, const R *__restrict__ ux,
+            const R *__restrict__ uy, const R *__restrict__ uz, double *partial)
+{
+    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        const double ph = (double)phi[idx];
+        a[0] += ph;
+        a[1] += (double)rho[idx];
+        if (ph >= 0.0) a[9] += 1.0;
+        if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2) continue;
+        const double vx = ux[idx], vy = uy[idx], vz = uz[idx];
+        const double usq = vx * vx + vy * vy + vz * vz;
+        a[6] = usq > a[6] ? usq : a[6];
+        Nbr nb(x, y, z, nx, ny, nz);
+        double p[19];
+#pragma unroll
+        for (int d = 0; d < 19; ++d) p[d] = (double)phi[nb.at(cx(d), cy(d), cz(d))];
+        const double phin2 = ph * ph;
+        const double lap = lap19(p);
+        const double gx = (2.0 * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] +
+                           p[13] - p[14]) * W<double>::inv12;
+        const double gy = (2.0 * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] +
+                           p[17] - p[18]) * W<double>::inv12;
+        const double gz = (2.0 * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] +
+                           p[18] - p[17]) * W<double>::inv12;
+        const double gsq = gx * gx + gy * gy + gz * gz;
+        const double pres = alpha * (phin2 * (3.0 * phin2 - 2.0 * phi_star_sq) -
+                                     phi_star_sq * phi_star_sq) -
+                            kappa * (ph * lap - 0.5 * gsq) + W<double>::cs_sq * (double)rho[idx];
+        if (ph >= 0.0) {
+            a[2] += 1.0;
+            a[3] += vx;
+            a[4] += vy;
+            a[5] += vz;
+            a[7] += pres;
+        } else {
+            a[8] += pres;
+        }
+    }
+    // wavefront reduction (64 lanes), then across the block's 4 waves
+    __shared__ double red[4][10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+        double v = a[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v, off, 64);
+            v = (j == 6) ? (o > v ? o : v) : v + o;
+        }
+        a[j] = v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int j = 0; j < 10; ++j) red[wv][j] = a[j];
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        const int j = threadIdx.x;
+        double v = red[0][j];
+        for (int w = 1; w < 4; ++w) v = (j == 6) ? (red[w][j] > v ? red[w][j] : v) : v + red[w][j];
+        partial[(int64_t)blockIdx.x * 10 + j] = v;
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(64) k_stats_final(int nblocks, const double *partial, double *out)
+{
+    const int j = threadIdx.x;
+    if (j >= 10) return;
+    double v = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+        const double p = partial[(int64_t)b * 10 + j];
+        v = (j == 6) ? (p > v ? p : v) : v + p;
+    }
+    out[j] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Ops<S> launchers
+// ---------------------------------------------------------------------------
+template <class S>
+hipError_t Ops<S>::lat_init(const Geo &g, const zsc_params &p, hipStream_t s, const void *phi,
+                            const void *rho, void *A)
+{
+    hipLaunchKernelGGL(k_lat_init<S>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                       make_coef<R>(p, 0), (const R *)phi, (const R *)rho,
+                       LatLayout<S>::view(A, g.n));
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::lat_phi(const Geo &g, hipStream_t s, const void *A, void *phi)
+{
+    zsc_params p{};
+    hipLaunchKernelGGL(k_lat_phi<S>, grid_for(g.n), dim3(256), 0, s, g.n, make_coef<R>(p, 0),
+                       LatLayout<S>::view(const_cast<void *>(A), g.n), (R *)phi);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::lat_collide(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
+                               const void *A, const void *phi, void *B)
+{
+    hipLaunchKernelGGL(k_lat_collide<S>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                       make_coef<R>(p, grav_on), LatLayout<S>::view(const_cast<void *>(A), g.n),
+                       (const R *)phi, LatLayout<S>::view(B, g.n));
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::lat_stream(const Geo &g, const zsc_params &p, hipStream_t s, const void *B,
+                              void *A)
+{
+    hipLaunchKernelGGL(k_lat_stream<S>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                       make_coef<R>(p, 0), LatLayout<S>::view(const_cast<void *>(B), g.n),
+                       LatLayout<S>::view(A, g.n));
+    return hipGetLastError();
+}
+
+template <class S, int TY>
+static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R> &c,
+                                  hipStream_t s, int zchunk, const void *Bin, void *Bout)
+{
+    const int nbx = (g.nx + 63) / 64, nby = (g.ny + TY - 1) / TY;
+    const int nbz = (g.nz + zchunk - 1) / zchunk;
+    const int64_t nblk = (int64_t)nbx * nby * nbz;
+    if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fused<S, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s, g.nx, g.ny,
+                       g.nz, g.n, zchunk, nbx, nby, (int)nblk, c,
+                       LatLayout<S>::view(const_cast<void *>(Bin), g.n),
+                       LatLayout<S>::view(Bout, g.n));
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::lat_fused(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
+                             const FusedCfg &cfg, const void *Bin, void *Bout)
+{
+    const Coef<R> c = make_coef<R>(p, grav_on);
+    const int zc = cfg.zchunk > 0 ? cfg.zchunk : g.nz;
+    switch (cfg.ty) {
+    case 1: return launch_fused_ty<S, 1>(g, c, s, zc, Bin, Bout);
+    case 2: return launch_fused_ty<S, 2>(g, c, s, zc, Bin, Bout);
+    case 4: return launch_fused_ty<S, 4>(g, c, s, zc, Bin, Bout);
+    case 8: return launch_fused_ty<S, 8>(g, c, s, zc, Bin, Bout);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class S>
+hipError_t Ops<S>::lat_moments(const Geo &g, const zsc_params &p, hipStream_t s, const void *A,
+                               void *phi, void *rho, void *ux, void *uy, void *uz)
+{
+    hipLaunchKernelGGL(k_lat_moments<S>, grid_for(g.n), dim3(256), 0, s, g.n, make_coef<R>(p, 0),
+                       LatLayout<S>::view(const_cast<void *>(A), g.n), (R *)phi, (R *)rho,
+                       (R *)ux, (R *)uy, (R *)uz);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::pop_get(const Geo &g, const zsc_params &p, hipStream_t s, const void *L, int q,
+                           double *dst)
+{
+    const Lattice<S> v = LatLayout<S>::view(const_cast<void *>(L), g.n);
+    const Coef<R> c = make_coef<R>(p, 0);
+    const S *ptr = q == 0 ? nullptr : (q < 7 ? v.f + (int64_t)(q - 1) * g.n : v.g + (int64_t)(q - 7) * g.n);
+    const R rest = q < 7 ? R(0) : g_weight<R>(q - 7) * c.rho_mean;
+    hipLaunchKernelGGL(k_pop_get<S>, grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, q == 0 ? 1 : 0,
+                       rest, dst);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::pop_set(const Geo &g, const zsc_params &p, hipStream_t s, void *L, int q,
+                           const double *src)
+{
+    const Lattice<S> v = LatLayout<S>::view(L, g.n);
+    const Coef<R> c = make_coef<R>(p, 0);
+    S *ptr = q == 0 ? nullptr : (q < 7 ? v.f + (int64_t)(q - 1) * g.n : v.g + (int64_t)(q - 7) * g.n);
+    const R rest = q < 7 ? R(0) : g_weight<R>(q - 7) * c.rho_mean;
+    hipLaunchKernelGGL(k_pop_set<S>, grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, q == 0 ? 1 : 0,
+                       rest, src);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::to_f64(hipStream_t s, int64_t n, const void *src, double *dst)
+{
+    hipLaunchKernelGGL(k_to_f64<R>, grid_for(n), dim3(256), 0, s, n, (const R *)src, dst);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::from_f64(hipStream_t s, int64_t n, const double *src, void *dst)
+{
+    hipLaunchKernelGGL(k_from_f64<R>, grid_for(n), dim3(256), 0, s, n, src, (R *)dst);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::init_phi_bubbles(const Geo &g, hipStream_t s, const double *bubbles, int nb,
+                                    double phi_star, double width, void *phi)
+{
+    hipLaunchKernelGGL(k_init_phi_bubbles<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.n,
+                       bubbles, nb, phi_star, width, (R *)phi);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::stats(const Geo &g, const zsc_params &p, hipStream_t s, const void *phi,
+                         const void *rho, const void *ux, const void *uy, const void *uz,
+                         double *partial, double *out)
+{
+    hipLaunchKernelGGL(k_stats<R>, dim3(kStatsBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                       p.alpha, p.kappa, p.phi_star_sq, (const R *)phi, (const R *)rho,
+                       (const R *)ux, (const R *)uy, (const R *)uz, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_stats_final<R>, dim3(1), dim3(64), 0, s, kStatsBlocks, partial, out);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Reference-layout (drop-in) kernels: each population pointer addresses
+// 2n elements, A = [0, n) post-stream, B = [n, 2n) post-collision.
+// ===========================================================================
+template <class T, int Q>
+struct Pops {
+    T *p[Q];
+};
+
+template <class T, int Q>
+inline Pops<T, Q> pops(T *const *a)
+{
+    Pops<T, Q> r;
+    for (int i = 0; i < Q; ++i) r.p[i] = a[i];
+    return r;
+}
+
+// src/init_f.cu:42-79
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_init_f(int nx, int ny, int nz, int64_t n, Coef<T> c, const T *__restrict__ phi, Pops<T, 7> f)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        T p[19], v[7];
+        gather19(phi, nb, p);
+        init_f_cell(c, p, v);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) f.p[i][idx] = v[i];
+    }
+}
+
+// src/init_g.cu:46-93
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_init_g(int nx, int ny, int nz, int64_t n, Coef<T> c, const T *__restrict__ phi,
+             const T *__restrict__ rho, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        T p[19], v[19];
+        gather19(phi, nb, p);
+        init_g_cell(c, p, rho[idx], v);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) g.p[i][idx] = v[i];
+    }
+}
+
+// src/update_phi.cu:40-44
+template <class T>
+__global__ void __launch_bounds__(256) k_update_phi(int64_t n, T *phi, Pops<T, 7> f)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        T v[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) v[i] = f.p[i][idx];
+        phi[idx] = sum_f(v);
+    }
+}
+
+// src/update_rho.cu:44-51
+template <class T>
+__global__ void __launch_bounds__(256) k_update_rho(int64_t n, T *rho, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        T v[19];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) v[i] = g.p[i][idx];
+        rho[idx] = sum_g(v);
+    }
+}
+
+// src/update_velocity.cu:45-61 (g_0 is not read)
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_update_velocity(int64_t n, const T *rho, T *ux, T *uy, T *uz, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        T v[19];
+        v[0] = T(0);
+#pragma unroll
+        for (int i = 1; i < 19; ++i) v[i] = g.p[i][idx];
+        T a, b, d;
+        velocity(v, rho[idx], a, b, d);
+        ux[idx] = a;
+        uy[idx] = b;
+        uz[idx] = d;
+    }
+}
+
+// src/collision_f.cu:44-116
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_collision_f(int nx, int ny, int nz, int64_t n, Coef<T> c, const T *__restrict__ phi,
+                  const T *__restrict__ rho, const T *__restrict__ ux, const T *__restrict__ uy,
+                  const T *__restrict__ uz, Pops<T, 7> f)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        T p[19], fa[7], fb[7], mu, fx, fy, fz;
+        gather19(phi, nb, p);
+        force19(c, p, mu, fx, fy, fz);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) fa[i] = f.p[i][idx];
+        collide_f(c, p[0], rho[idx], ux[idx], uy[idx], uz[idx], mu, fx, fy, fz, fa, fb);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) f.p[i][n + idx] = fb[i];
+    }
+}
+
+// src/collision_g.cu:49-196
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_collision_g(int nx, int ny, int nz, int64_t n, Coef<T> c, const T *__restrict__ phi,
+                  const T *__restrict__ rho, const T *__restrict__ ux, const T *__restrict__ uy,
+                  const T *__restrict__ uz, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        T p[19], ga[19], gb[19], mu, fx, fy, fz;
+        gather19(phi, nb, p);
+        force19(c, p, mu, fx, fy, fz);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) ga[i] = g.p[i][idx];
+        collide_g(c, p[0], rho[idx], ux[idx], uy[idx], uz[idx], mu, fx, fy, fz, ga, gb);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) g.p[i][n + idx] = gb[i];
+    }
+}
+
+// src/stream_f.cu:40-61 as a pull, halo of src/pack_mpi_f.cu / unpack_mpi_f.cu
+// and src/pack_mpi.cu / unpack_mpi.cu folded into the periodic wrap.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_stream_f(int nx, int ny, int nz, int64_t n, T eta, T eta2, Pops<T, 7> f)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        f.p[0][idx] = f.p[0][n + idx];
+#pragma unroll
+        for (int i = 1; i < 7; ++i)
+            f.p[i][idx] = eta * f.p[i][n + nb.at(-cx(i), -cy(i), -cz(i))] + eta2 * f.p[i][n + idx];
+    }
+}
+
+// src/stream_g.cu:44-78 as a pull (+ halo, as above).
+template <class T>
+__global__ void __launch_bounds__(256) k_stream_g(int nx, int ny, int nz, int64_t n, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) g.p[i][idx] = g.p[i][n + nb.at(-cx(i), -cy(i), -cz(i))];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_export(int64_t n, Lattice<T> A, Pops<T, 7> f, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        f.p[0][idx] = A.f0[idx];
+#pragma unroll
+        for (int i = 1; i < 7; ++i) f.p[i][idx] = A.f[(int64_t)(i - 1) * n + idx];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) g.p[i][idx] = A.g[(int64_t)i * n + idx];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_import(int64_t n, Lattice<T> A, Pops<T, 7> f, Pops<T, 19> g)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        A.f0[idx] = f.p[0][idx];
+#pragma unroll
+        for (int i = 1; i < 7; ++i) A.f[(int64_t)(i - 1) * n + idx] = f.p[i][idx];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) A.g[(int64_t)i * n + idx] = g.p[i][idx];
+    }
+}
+
+#define ZSC_LAUNCH(kern, ...)                                                     \
+    do {                                                                          \
+        hipLaunchKernelGGL(kern, grid_for(g.n), dim3(256), 0, s, __VA_ARGS__);   \
+        return hipGetLastError();                                                 \
+    } while (0)
+
+template <class T>
+hipError_t DropIn<T>::init_f(const Geo &g, const zsc_params &p, hipStream_t s, const T *phi,
+                             T *const f[7])
+{
+    ZSC_LAUNCH(k_init_f<T>, g.nx, g.ny, g.nz, g.n, make_coef<T>(p, 0), phi, pops<T, 7>(f));
+}
+
+template <class T>
+hipError_t DropIn<T>::init_g(const Geo &g, const zsc_params &p, hipStream_t s, const T *phi,
+                             const T *rho, T *const gp[19])
+{
+    ZSC_LAUNCH(k_init_g<T>, g.nx, g.ny, g.nz, g.n, make_coef<T>(p, 0), phi, rho, pops<T, 19>(gp));
+}
+
+template <class T>
+hipError_t DropIn<T>::update_phi(const Geo &g, hipStream_t s, T *phi, T *const f[7])
+{
+    ZSC_LAUNCH(k_update_phi<T>, g.n, phi, pops<T, 7>(f));
+}
+
+template <class T>
+hipError_t DropIn<T>::update_rho(const Geo &g, hipStream_t s, T *rho, T *const gp[19])
+{
+    ZSC_LAUNCH(k_update_rho<T>, g.n, rho, pops<T, 19>(gp));
+}
+
+template <class T>
+hipError_t DropIn<T>::update_velocity(const Geo &g, hipStream_t s, const T *rho, T *ux, T *uy,
+                                      T *uz, T *const gp[19])
+{
+    ZSC_LAUNCH(k_update_velocity<T>, g.n, rho, ux, uy, uz, pops<T, 19>(gp));
+}
+
+template <class T>
+hipError_t DropIn<T>::collision_f(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
+                                  const T *phi, const T *rho, const T *ux, const T *uy,
+                                  const T *uz, T *const f[7])
+{
+    ZSC_LAUNCH(k_collision_f<T>, g.nx, g.ny, g.nz, g.n, make_coef<T>(p, grav_on), phi, rho, ux,
+               uy, uz, pops<T, 7>(f));
+}
+
+template <class T>
+hipError_t DropIn<T>::collision_g(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
+                                  const T *phi, const T *rho, const T *ux, const T *uy,
+                                  const T *uz, T *const gp[19])
+{
+    ZSC_LAUNCH(k_collision_g<T>, g.nx, g.ny, g.nz, g.n, make_coef<T>(p, grav_on), phi, rho, ux,
+               uy, uz, pops<T, 19>(gp));
+}
+
+template <class T>
+hipError_t DropIn<T>::stream_f(const Geo &g, const zsc_params &p, hipStream_t s, T *const f[7])
+{
+    ZSC_LAUNCH(k_stream_f<T>, g.nx, g.ny, g.nz, g.n, (T)p.eta, (T)p.eta2, pops<T, 7>(f));
+}
+
+template <class T>
+hipError_t DropIn<T>::stream_g(const Geo &g, hipStream_t s, T *const gp[19])
+{
+    ZSC_LAUNCH(k_stream_g<T>, g.nx, g.ny, g.nz, g.n, pops<T, 19>(gp));
+}
+
+template <class T>
+hipError_t DropIn<T>::export_state(const Geo &g, hipStream_t s, const void *A, T *const f[7],
+                                   T *const gp[19])
+{
+    ZSC_LAUNCH(k_export<T>, g.n, LatLayout<T>::view(const_cast<void *>(A), g.n), pops<T, 7>(f),
+               pops<T, 19>(gp));
+}
+
+template <class T>
+hipError_t DropIn<T>::import_state(const Geo &g, hipStream_t s, void *A, T *const f[7],
+                                   T *const gp[19])
+{
+    ZSC_LAUNCH(k_import<T>, g.n, LatLayout<T>::view(A, g.n), pops<T, 7>(f), pops<T, 19>(gp));
+}
+
+#undef ZSC_LAUNCH
+
+}  // namespace zsc

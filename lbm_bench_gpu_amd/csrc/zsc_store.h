@@ -1,0 +1,98 @@
+// zsc_store.h -- population storage codecs.
+//
+// FP64 and FP32 lattices store populations as they are.  The 16-bit
+// lattices store the deviation from a per-population rest value (g_i:
+// w_i * rho_mean; f_1..6: 0) rounded to nearest-even, and decode to FP32 for
+// compute.  f_0 ~ phi is O(1) with no useful rest value, so the 16-bit
+// lattices keep f_0 in FP32 (its own array, type S0 below).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zsc {
+
+struct bf16s {
+    uint16_t v;
+};
+struct f16s {
+    uint16_t v;
+};
+
+template <class S>
+struct Codec;
+
+template <>
+struct Codec<double> {
+    static constexpr bool deviation = false;
+    __host__ __device__ static double dec(double s, double) { return s; }
+    __host__ __device__ static double enc(double v, double) { return v; }
+};
+
+template <>
+struct Codec<float> {
+    static constexpr bool deviation = false;
+    __host__ __device__ static float dec(float s, float) { return s; }
+    __host__ __device__ static float enc(float v, float) { return v; }
+};
+
+// bf16: upper half of the FP32 pattern, rounded to nearest-even; NaN kept
+// quiet so divergence still shows in the reductions.
+__host__ __device__ inline uint16_t f32_to_bf16_rne(float f)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+template <>
+struct Codec<bf16s> {
+    static constexpr bool deviation = true;
+    __host__ __device__ static float dec(bf16s s, float rest) { return bf16_to_f32(s.v) + rest; }
+    __host__ __device__ static bf16s enc(float v, float rest) { return {f32_to_bf16_rne(v - rest)}; }
+};
+
+template <>
+struct Codec<f16s> {
+    static constexpr bool deviation = true;
+    __host__ __device__ static float dec(f16s s, float rest)
+    {
+        _Float16 h;
+        __builtin_memcpy(&h, &s.v, 2);
+        return (float)h + rest;
+    }
+    __host__ __device__ static f16s enc(float v, float rest)
+    {
+        _Float16 h = (_Float16)(v - rest);
+        f16s s;
+        __builtin_memcpy(&s.v, &h, 2);
+        return s;
+    }
+};
+
+// Compute type and f_0 storage type for a population storage type S.
+template <class S> struct Traits;
+template <> struct Traits<double> { using R = double; using S0 = double; };
+template <> struct Traits<float> { using R = float; using S0 = float; };
+template <> struct Traits<bf16s> { using R = float; using S0 = float; };
+template <> struct Traits<f16s> { using R = float; using S0 = float; };
+
+// One lattice (A or B) in the context's layout: f_0 array, f_1..6 with
+// stride n, g_0..18 with stride n.  Pointers are device addresses.
+template <class S>
+struct Lattice {
+    typename Traits<S>::S0 *f0;
+    S *f;  // f_i at f + (i-1)*n, i = 1..6
+    S *g;  // g_i at g + i*n
+};
+
+}  // namespace zsc

@@ -1,0 +1,92 @@
+/*
+ * zsc_oracle.h -- FP64 CPU restatement of the Zheng-Shu-Chew (ZSC) bubbly-flow
+ * lattice-Boltzmann step of carlosrosales/lbm_bench_gpu.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it; the product (libzsc.so) never links or calls it.
+ *
+ * Layout: a fully periodic nx*ny*nz box without ghost planes, cell index
+ * idx = x + nx*(y + ny*z) (x fastest, as src/gridid.c:32-35), populations
+ * stored pop-major: pop i of cell idx lives at [i*N + idx], N = nx*ny*nz.
+ * z is stored 0-based; the reference's real plane k=1..zmax is z = k-1.
+ */
+#ifndef ZSC_ORACLE_H
+#define ZSC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scalars used by the step; names follow src/constants.h:62-85. */
+typedef struct zo_coef {
+    double phi_star, phi_star_sq, rho_mean;
+    double alpha, alpha4, kappa, gamma;
+    double eta, eta2, inv_eta2;
+    double inv_tau_phi, inv_tau_phi_one;
+    double inv_tau_rho_one;
+    double k0, k1, k2, kc0, kc1, kc2;
+    double gravity, grav;      /* grav = 2*phi_star*gravity, applied when on */
+    double width, sigma;
+} zo_coef;
+
+/* Parameter derivation of src/paramread.c:141-170 in FP64.  r0 is the radius
+ * of bubble 0 (src/paramread.c:161).  phi_star_override > 0 replaces
+ * 0.5*(rho_h-rho_l); gravity_override >= 0 replaces the Eo-based gravity. */
+void zo_derive(double rho_l, double rho_h, double tau_rho, double tau_phi,
+               double width, double sigma, double gamma, double eo, double r0,
+               double phi_star_override, double gravity_override,
+               zo_coef *c);
+
+/* Initial order parameter of src/init.c:39-59 (tanh bubbles, last one wins,
+ * z 1-based).  bubbles is [nb][4] = xc, yc, zc, R. */
+void zo_init_phi(int nx, int ny, int nz, const double *bubbles, int nb,
+                 double phi_star, double width, double *phi);
+
+void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const double *phi,
+               double *f);
+void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const double *phi,
+               const double *rho, double *g);
+
+void zo_update_phi(int nx, int ny, int nz, const double *f, double *phi);
+void zo_update_rho(int nx, int ny, int nz, const double *g, double *rho);
+void zo_update_velocity(int nx, int ny, int nz, const double *g,
+                        const double *rho, double *ux, double *uy, double *uz);
+
+/* A -> B collisions; grav_on selects grav (evolution) or 0 (relaxation). */
+void zo_collision_f(int nx, int ny, int nz, const zo_coef *c, int grav_on,
+                    const double *phi, const double *rho, const double *ux,
+                    const double *uy, const double *uz, const double *fa,
+                    double *fb);
+void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
+                    const double *phi, const double *rho, const double *ux,
+                    const double *uy, const double *uz, const double *ga,
+                    double *gb);
+
+/* B -> A streaming with periodic wrap. */
+void zo_stream_f(int nx, int ny, int nz, const zo_coef *c, const double *fb,
+                 double *fa);
+void zo_stream_g(int nx, int ny, int nz, const double *gb, double *ga);
+
+/* nsteps full steps in the reference order (src/main.cu:496-590).  f, g hold
+ * the A lattice on entry and exit; work must hold 7N + 19N + 5N doubles. */
+void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
+            double *f, double *g, double *work);
+
+/* Diagnostics (src/stats.c:39-55, src/logsave.c:63-147) over moments.
+ * out[0]  sum phi (all cells)        out[1]  sum rho (all cells)
+ * out[2]  bubble volume (phi>=0 count over i,j in [1,n-2], all z)
+ * out[3..5] sum of ux,uy,uz over those bubble cells
+ * out[6]  max |u|^2 over i,j in [1,n-2]
+ * out[7]  pressure sum inside (phi>=0)   out[8] pressure sum outside
+ * out[9]  phi>=0 count over the full box */
+void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
+              const double *rho, const double *ux, const double *uy,
+              const double *uz, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""GPU parity: libzsc's HIP kernels against the FP64 oracle, through the C-ABI.
+
+Bar: FP64 contexts match the oracle to 1e-12 relative error (max |diff| over
+max |oracle| per field) on f, g, phi, rho and u; FP32 and the 16-bit storage
+modes are checked against the FP64 oracle with the tolerances stated per test.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from lbm_bench_gpu_amd import configs, zsc
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def oracle_coef(p: zsc.Params):
+    c = O.ZoCoef()
+    for k, _ in O.ZoCoef._fields_:
+        setattr(c, k, p.values[k])
+    return c
+
+
+def small_case(dims, nb=2, seed=7):
+    """A box with overlapping tanh bubbles near the faces (wrap exercised)."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    b = np.column_stack([rng.uniform(0, nx, nb), rng.uniform(0, ny, nb),
+                         rng.uniform(1, nz + 1, nb), rng.uniform(2, 4, nb)])
+    inp = configs.physics(*dims)
+    p = zsc.derive_params(inp, float(b[0, 3]))
+    phi0 = O.init_phi((nz, ny, nx), b, p.phi_star, p.width)
+    # perturb so that every population and velocity is non-trivial
+    phi0 = phi0 + 0.05 * rng.standard_normal(phi0.shape)
+    return p, phi0
+
+
+def run_oracle(p, phi0, steps, grav=True):
+    st = O.State(phi0.shape, oracle_coef(p)).init(phi0)
+    st.run(steps, grav_on=grav)
+    return st
+
+
+def assert_fields(ctx, st, tol, label):
+    f, g = ctx.state_get()
+    errs = {"f": rel(f, st.f), "g": rel(g, st.g)}
+    m_gpu = ctx.moments()
+    m_ref = st.moments()
+    for name, a, b in zip(("phi", "rho", "ux", "uy", "uz"), m_gpu, m_ref):
+        errs[name] = rel(a, b)
+    print(label, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"{label}: {bad} above {tol}"
+
+
+@pytest.mark.parametrize("dims,cfg", [
+    ((16, 16, 16), (4, 64)),
+    ((20, 12, 10), (4, 3)),     # partial tiles in x and y, z chunks of 3
+    ((70, 9, 5), (8, 2)),       # two x tiles, one partial
+    ((8, 8, 8), (1, 1)),
+    ((33, 17, 6), (2, 0)),
+])
+def test_fused_f64_bitwise_small(need_gpu, dims, cfg):
+    p, phi0 = small_case(dims)
+    st = run_oracle(p, phi0, 12)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.set_fused_config(*cfg)
+        ctx.state_init(phi0)
+        ctx.step(5)
+        ctx.step(7)  # resumes from the lazily-held post-collision state
+        assert_fields(ctx, st, 0.0, f"fused f64 {dims} cfg={cfg}")
+
+
+def test_fused_f64_relaxation_then_gravity(need_gpu):
+    dims = (24, 16, 12)
+    p, phi0 = small_case(dims, seed=3)
+    st = O.State(phi0.shape, oracle_coef(p)).init(phi0)
+    st.run(6, grav_on=False)
+    st.run(6, grav_on=True)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.set_gravity(False)
+        ctx.step(6)
+        ctx.set_gravity(True)
+        ctx.step(6)
+        assert_fields(ctx, st, 0.0, "relax+grav")
+
+
+def test_config0_fp64_parity(need_gpu):
+    """64^3, one R=10 bubble, FP64, 1000 steps: within 1e-12 of the oracle."""
+    w = configs.workload(0)
+    p = w.params()
+    nx, ny, nz = w.dims
+    phi0 = O.init_phi((nz, ny, nx), w.bubbles, p.phi_star, p.width)
+    st = run_oracle(p, phi0, w.steps)
+    with zsc.Context(*w.dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        s0 = ctx.stats()
+        ctx.step(w.steps)
+        assert_fields(ctx, st, FP64_TOL, "config0 f64")
+        s1 = ctx.stats()
+    # exact conservation of total phi and mass, to round-off
+    n = nx * ny * nz
+    assert abs(s1["sum_phi"] - s0["sum_phi"]) <= 1e-13 * n
+    assert abs(s1["sum_rho"] - s0["sum_rho"]) <= 1e-13 * n * p.rho_mean
+
+
+def _dropin_tensors(n, dtype):
+    import torch
+    dev = "cuda"
+    f = [torch.zeros(2 * n, dtype=dtype, device=dev) for _ in range(7)]
+    g = [torch.zeros(2 * n, dtype=dtype, device=dev) for _ in range(19)]
+    mom = [torch.zeros(n, dtype=dtype, device=dev) for _ in range(5)]
+    return f, g, mom
+
+
+def reference_sequence(ctx, f, g, mom, steps):
+    """The per-step launch order of src/main.cu:498-566."""
+    phi, rho, ux, uy, uz = mom
+    for _ in range(steps):
+        ctx.update_phi(phi, f)
+        ctx.update_rho(rho, g)
+        ctx.update_velocity(rho, ux, uy, uz, g)
+        ctx.collision_f(phi, rho, ux, uy, uz, f)
+        ctx.collision_g(phi, rho, ux, uy, uz, g)
+        ctx.stream_f(f)
+        ctx.stream_g(g)
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 10), (32, 32, 8)])
+def test_dropin_f64_matches_oracle(need_gpu, dims):
+    import torch
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, seed=11)
+    st = run_oracle(p, phi0, 9)
+    f, g, mom = _dropin_tensors(n, torch.float64)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        mom[0].copy_(torch.from_numpy(phi0.ravel()))
+        mom[1].fill_(p.rho_mean)
+        ctx.init_f(mom[0], f)
+        ctx.init_g(mom[0], mom[1], g)
+        reference_sequence(ctx, f, g, mom, 9)
+        ctx.sync()
+        fa = np.stack([t[:n].cpu().numpy() for t in f]).reshape(7, nz, ny, nx)
+        ga = np.stack([t[:n].cpu().numpy() for t in g]).reshape(19, nz, ny, nx)
+        assert rel(fa, st.f) == 0.0 and rel(ga, st.g) == 0.0, (rel(fa, st.f), rel(ga, st.g))
+        # the drop-in state imports into the fused path and continues identically
+        ctx.state_import(f, g)
+        ctx.step(3)
+        st.run(3)
+        assert_fields(ctx, st, 0.0, "import+fused")
+        # and exports back into reference-layout arrays
+        ctx.state_export(f, g)
+        ctx.sync()
+        fa = np.stack([t[:n].cpu().numpy() for t in f]).reshape(7, nz, ny, nx)
+        assert rel(fa, st.f) == 0.0
+
+
+def test_dropin_f32_close_to_oracle(need_gpu):
+    import torch
+    dims = (16, 16, 16)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, seed=5)
+    st = run_oracle(p, phi0, 20)
+    f, g, mom = _dropin_tensors(n, torch.float32)
+    with zsc.Context(*dims, precision="f32") as ctx:
+        ctx.set_params(p)
+        mom[0].copy_(torch.from_numpy(phi0.ravel()).float())
+        mom[1].fill_(p.rho_mean)
+        ctx.init_f(mom[0], f)
+        ctx.init_g(mom[0], mom[1], g)
+        reference_sequence(ctx, f, g, mom, 20)
+        ctx.sync()
+        fa = np.stack([t[:n].double().cpu().numpy() for t in f]).reshape(7, nz, ny, nx)
+        # FP32 rounding over 20 steps: 1e-5 of the field scale
+        assert rel(fa, st.f) < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol_phi", [("f32", 2e-5), ("bf16", 2e-2), ("fp16", 2e-3)])
+def test_reduced_precision_vs_fp64(need_gpu, prec, tol_phi):
+    dims = (32, 32, 32)
+    nx, ny, nz = dims
+    b = np.array([[16.0, 16.0, 16.0, 8.0]])
+    p = zsc.derive_params(configs.physics(*dims), 8.0)
+    phi0 = O.init_phi((nz, ny, nx), b, p.phi_star, p.width)
+    st = run_oracle(p, phi0, 100)
+    with zsc.Context(*dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(100)
+        phi, rho, ux, uy, uz = ctx.moments()
+        st_phi, st_rho, *_ = st.moments()
+        e_phi = np.abs(phi - st_phi).max()
+        print(prec, "max|dphi|", e_phi, "max|drho|", np.abs(rho - st_rho).max(), flush=True)
+        assert e_phi < tol_phi
+
+
+def test_stats_match_oracle(need_gpu):
+    dims = (24, 20, 16)
+    p, phi0 = small_case(dims, seed=2)
+    st = run_oracle(p, phi0, 10)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(10)
+        s = ctx.stats()
+    ref = O.stats(oracle_coef(p), *st.moments())
+    names = ["sum_phi", "sum_rho", "bubble_volume", "ux_sum", "uy_sum", "uz_sum", "umax_sq",
+             "p_in_sum", "p_out_sum", "bubble_volume_full"]
+    for k, r in zip(names, ref):
+        assert abs(s[k] - r) <= 1e-10 * max(1.0, abs(r)), (k, s[k], r)
+
+
+def test_bubble_init_on_device_matches_host(need_gpu):
+    dims = (48, 40, 36)
+    nx, ny, nz = dims
+    b = zsc.generate_bubbles(9, 3, 4, 6, 2.0, 4.0, dims)
+    p = zsc.derive_params(configs.physics(*dims), float(b[0, 3]))
+    phi_h = zsc.init_phi_host(dims, b, p.phi_star, p.width)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init_bubbles(b)
+        f, g = ctx.state_get()
+    st = O.State((nz, ny, nx), oracle_coef(p)).init(phi_h)
+    # device tanh may differ from libm by an ulp
+    assert rel(f, st.f) < 1e-13 and rel(g, st.g) < 1e-13
