@@ -216,10 +216,13 @@ typedef struct zsc_stats {
  * is NaN or Inf. */
 int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
 
-/* Tile shape of the fused kernel: ty wavefront rows of 64 cells per block
- * (1, 2, 4 or 8) and zchunk planes marched per block (0: the whole box).
- * No reference counterpart; a tuning knob of this implementation. */
-int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk);
+/* Launch shape of the fused kernel: ty wavefront rows of 64 cells per block
+ * (1, 2, 4 or 8), zchunk planes marched per block (0: the whole box) and
+ * flags: 1 non-temporal output stores, 2 (with 1) non-temporal loads, 4 cap
+ * registers for 4 waves per SIMD, 8 no XCD-contiguous tile order.  Flags
+ * only apply to the FP32-compute precisions.  No reference counterpart; a
+ * tuning knob of this implementation. */
+int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);
 
 /* Bytes per lattice update the fused step must move (26 populations read
  * and written once at the storage precision). */

@@ -65,7 +65,7 @@ SIGNATURES = {
     "zsc_destroy": (_I, [_VP]),
     "zsc_set_params": (_I, [_VP, ctypes.POINTER(ZscParams)]),
     "zsc_set_gravity": (_I, [_VP, _I]),
-    "zsc_set_fused_config": (_I, [_VP, _I, _I]),
+    "zsc_set_fused_config": (_I, [_VP, _I, _I, _I]),
     "zsc_sync": (_I, [_VP]),
     "zsc_last_error": (ctypes.c_char_p, []),
     "zsc_init_f": (_I, [_VP, _VP, _PP7]),

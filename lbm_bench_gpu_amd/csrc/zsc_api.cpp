@@ -85,7 +85,7 @@ struct zsc_ctx {
     double *dbuf = nullptr;  // n doubles of staging for host transfers
     double *stats_buf = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    FusedCfg cfg{4, 64};
+    FusedCfg cfg{4, 16, 0};
 
     void *w(int i) const { return (char *)work + (size_t)i * align256((size_t)geo.n * rsize); }
 };
@@ -167,12 +167,13 @@ extern "C" int zsc_set_gravity(zsc_ctx *c, int on)
     return ZSC_OK;
 }
 
-extern "C" int zsc_set_fused_config(zsc_ctx *c, int ty, int zchunk)
+extern "C" int zsc_set_fused_config(zsc_ctx *c, int ty, int zchunk, int flags)
 {
     if (int r = need_ctx(c)) return r;
-    if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || zchunk < 0)
-        return fail(ZSC_ERR_ARG, "zsc_set_fused_config: ty in {1,2,4,8}, zchunk >= 0");
-    c->cfg = {ty, zchunk};
+    if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || zchunk < 0 || flags < 0 || flags > 15 ||
+        (flags & 3) == kFusedNtLoad)
+        return fail(ZSC_ERR_ARG, "zsc_set_fused_config: ty in {1,2,4,8}, zchunk >= 0, valid flags");
+    c->cfg = {ty, zchunk, flags};
     return ZSC_OK;
 }
 

@@ -99,15 +99,47 @@ inline dim3 grid_for(int64_t n, int block = 256)
 // Context-lattice kernels
 // ===========================================================================
 
-template <class S, class R>
+// Memory-access flags of the fused kernel: streaming (non-temporal) hints.
+enum : int { kNtStore = 1, kNtLoad = 2 };
+
+template <int FL, class S>
+__device__ inline S ld_raw(const S *p)
+{
+    if constexpr ((FL & kNtLoad) != 0) {
+        if constexpr (sizeof(S) == 2) {
+            S v;
+            v.v = __builtin_nontemporal_load((const uint16_t *)p);
+            return v;
+        } else {
+            return __builtin_nontemporal_load(p);
+        }
+    } else {
+        return *p;
+    }
+}
+
+template <int FL, class S>
+__device__ inline void st_raw(S *p, S v)
+{
+    if constexpr ((FL & kNtStore) != 0) {
+        if constexpr (sizeof(S) == 2)
+            __builtin_nontemporal_store(v.v, (uint16_t *)p);
+        else
+            __builtin_nontemporal_store(v, p);
+    } else {
+        *p = v;
+    }
+}
+
+template <class S, class R, int FL = 0>
 __device__ inline R ldg(const S *p, int64_t i, R rest)
 {
-    return Codec<S>::dec(p[i], rest);
+    return Codec<S>::dec(ld_raw<FL>(p + i), rest);
 }
-template <class S, class R>
+template <class S, class R, int FL = 0>
 __device__ inline void stg(S *p, int64_t i, R v, R rest)
 {
-    p[i] = Codec<S>::enc(v, rest);
+    st_raw<FL>(p + i, Codec<S>::enc(v, rest));
 }
 
 template <class R>
@@ -128,40 +160,41 @@ __device__ inline void load_local(const Lattice<S> &L, int64_t n, int64_t idx, c
     for (int i = 0; i < 19; ++i) g[i] = ldg(L.g, (int64_t)i * n + idx, g_rest(c, i));
 }
 
-template <class S, class R>
+template <class S, class R, int FL = 0>
 __device__ inline void store_local(const Lattice<S> &L, int64_t n, int64_t idx, const Coef<R> &c,
                                    const R *f, const R *g)
 {
-    stg(L.f0, idx, f[0], R(0));
+    stg<typename Traits<S>::S0, R, FL>(L.f0, idx, f[0], R(0));
 #pragma unroll
-    for (int i = 1; i < 7; ++i) stg(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
+    for (int i = 1; i < 7; ++i) stg<S, R, FL>(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
 #pragma unroll
-    for (int i = 0; i < 19; ++i) stg(L.g, (int64_t)i * n + idx, g[i], g_rest(c, i));
+    for (int i = 0; i < 19; ++i) stg<S, R, FL>(L.g, (int64_t)i * n + idx, g[i], g_rest(c, i));
 }
 
 // Post-stream f of one cell, pulled from a B lattice (src/stream_f.cu:52-59
 // as a pull): f_A,i(y) = eta f_B,i(y - c_i) + eta2 f_B,i(y).
-template <class S, class R>
+template <class S, class R, int FL = 0>
 __device__ inline void pull_f(const Lattice<S> &B, int64_t n, const Nbr &nb, const Coef<R> &c,
                               R *f)
 {
     const int64_t own = nb.at(0, 0, 0);
-    f[0] = ldg(B.f0, own, R(0));
+    f[0] = ldg<typename Traits<S>::S0, R, FL>(B.f0, own, R(0));
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
         const S *p = B.f + (int64_t)(i - 1) * n;
-        f[i] = c.eta * ldg(p, nb.at(-cx(i), -cy(i), -cz(i)), R(0)) + c.eta2 * ldg(p, own, R(0));
+        f[i] = c.eta * ldg<S, R, FL>(p, nb.at(-cx(i), -cy(i), -cz(i)), R(0)) +
+               c.eta2 * ldg<S, R, FL>(p, own, R(0));
     }
 }
 
 // Post-stream g of one cell (src/stream_g.cu:56-76 as a pull).
-template <class S, class R>
+template <class S, class R, int FL = 0>
 __device__ inline void pull_g(const Lattice<S> &B, int64_t n, const Nbr &nb, const Coef<R> &c,
                               R *g)
 {
 #pragma unroll
     for (int i = 0; i < 19; ++i)
-        g[i] = ldg(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
+        g[i] = ldg<S, R, FL>(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
 }
 
 template <class S>
@@ -339,9 +372,9 @@ __global__ void __launch_bounds__(256)
 // sufficient: a wave writing slot (k+2)&3 can only run ahead of a wave that
 // still reads slots k-1..k+1 by one iteration.
 // ---------------------------------------------------------------------------
-template <class S, int TY>
-__global__ void __launch_bounds__(64 * TY)
-    k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk,
+template <class S, int TY, int FL, int MINW>
+__global__ void __launch_bounds__(64 * TY, MINW)
+    k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
             Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out)
 {
     using R = typename Traits<S>::R;
@@ -351,7 +384,7 @@ __global__ void __launch_bounds__(64 * TY)
     // contiguous range of tiles so that neighbouring rows share its L2.
     const int b = blockIdx.x;
     int l = b;
-    if ((nblk & 7) == 0) l = (b & 7) * (nblk >> 3) + (b >> 3);
+    if (remap && (nblk & 7) == 0) l = (b & 7) * (nblk >> 3) + (b >> 3);
     const int bx = l % nbx;
     const int by = (l / nbx) % nby;
     const int bz = l / (nbx * nby);
@@ -370,7 +403,7 @@ __global__ void __launch_bounds__(64 * TY)
         const int slot = (zz + 4) & 3;
         if (own) {
             Nbr nb(x, y, zc, nx, ny, nz);
-            pull_f(in, n, nb, c, fown);
+            pull_f<S, R, FL>(in, n, nb, c, fown);
             sp[slot][ty + 1][tx + 1] = sum_f(fown);
         }
         for (int h = tid; h < nh; h += 64 * TY) {
@@ -400,7 +433,7 @@ __global__ void __launch_bounds__(64 * TY)
             hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
             Nbr nb(hx, hy, zc, nx, ny, nz);
             R f[7];
-            pull_f(in, n, nb, c, f);
+            pull_f<S, R, FL>(in, n, nb, c, f);
             sp[slot][r][col] = sum_f(f);
         }
     };
@@ -411,7 +444,7 @@ __global__ void __launch_bounds__(64 * TY)
     for (int k = z0; k < z1; ++k) {
         R ga[19];
         Nbr nb(x, y, k, nx, ny, nz);
-        if (own) pull_g(in, n, nb, c, ga);
+        if (own) pull_g<S, R, FL>(in, n, nb, c, ga);
         plane_phi(k + 1, fnext);
         __syncthreads();
         if (own) {
@@ -439,7 +472,7 @@ __global__ void __launch_bounds__(64 * TY)
             p[18] = sp[spl][r - 1][q];
             R fb[7], gb[19];
             collide_cell(c, p, fcur, ga, fb, gb);
-            store_local(out, n, nb.at(0, 0, 0), c, fb, gb);
+            store_local<S, R, FL>(out, n, nb.at(0, 0, 0), c, fb, gb);
         }
 #pragma unroll
         for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
@@ -571,19 +604,42 @@ hipError_t Ops<S>::lat_stream(const Geo &g, const zsc_params &p, hipStream_t s, 
     return hipGetLastError();
 }
 
-template <class S, int TY>
-static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R> &c,
-                                  hipStream_t s, int zchunk, const void *Bin, void *Bout)
+template <class S, int TY, int FL, int MINW>
+static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R> &c,
+                                 hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout)
 {
+    const int zchunk = cfg.zchunk > 0 ? cfg.zchunk : g.nz;
     const int nbx = (g.nx + 63) / 64, nby = (g.ny + TY - 1) / TY;
     const int nbz = (g.nz + zchunk - 1) / zchunk;
     const int64_t nblk = (int64_t)nbx * nby * nbz;
     if (nblk > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_fused<S, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s, g.nx, g.ny,
-                       g.nz, g.n, zchunk, nbx, nby, (int)nblk, c,
+    hipLaunchKernelGGL((k_fused<S, TY, FL, MINW>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
+                       g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk,
+                       (cfg.flags & kFusedIdentityMap) ? 0 : 1, c,
                        LatLayout<S>::view(const_cast<void *>(Bin), g.n),
                        LatLayout<S>::view(Bout, g.n));
     return hipGetLastError();
+}
+
+template <class S, int TY>
+static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R> &c,
+                                  hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout)
+{
+    if constexpr (sizeof(S) == 8) {
+        return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout);
+    } else {
+        const int fl = cfg.flags & (kNtStore | kNtLoad);
+        const bool occ = (cfg.flags & kFusedOcc4) != 0;
+        switch (fl + (occ ? 4 : 0)) {
+        case 0: return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout);
+        case 1: return launch_fused_v<S, TY, 1, 1>(g, c, s, cfg, Bin, Bout);
+        case 3: return launch_fused_v<S, TY, 3, 1>(g, c, s, cfg, Bin, Bout);
+        case 4: return launch_fused_v<S, TY, 0, 4>(g, c, s, cfg, Bin, Bout);
+        case 5: return launch_fused_v<S, TY, 1, 4>(g, c, s, cfg, Bin, Bout);
+        case 7: return launch_fused_v<S, TY, 3, 4>(g, c, s, cfg, Bin, Bout);
+        default: return hipErrorInvalidValue;
+        }
+    }
 }
 
 template <class S>
@@ -591,12 +647,11 @@ hipError_t Ops<S>::lat_fused(const Geo &g, const zsc_params &p, int grav_on, hip
                              const FusedCfg &cfg, const void *Bin, void *Bout)
 {
     const Coef<R> c = make_coef<R>(p, grav_on);
-    const int zc = cfg.zchunk > 0 ? cfg.zchunk : g.nz;
     switch (cfg.ty) {
-    case 1: return launch_fused_ty<S, 1>(g, c, s, zc, Bin, Bout);
-    case 2: return launch_fused_ty<S, 2>(g, c, s, zc, Bin, Bout);
-    case 4: return launch_fused_ty<S, 4>(g, c, s, zc, Bin, Bout);
-    case 8: return launch_fused_ty<S, 8>(g, c, s, zc, Bin, Bout);
+    case 1: return launch_fused_ty<S, 1>(g, c, s, cfg, Bin, Bout);
+    case 2: return launch_fused_ty<S, 2>(g, c, s, cfg, Bin, Bout);
+    case 4: return launch_fused_ty<S, 4>(g, c, s, cfg, Bin, Bout);
+    case 8: return launch_fused_ty<S, 8>(g, c, s, cfg, Bin, Bout);
     default: return hipErrorInvalidValue;
     }
 }

@@ -34,9 +34,16 @@ struct LatLayout {
 };
 
 // Launch configuration of the fused kernel (tunable).
+enum : int {
+    kFusedNtStore = 1,      // non-temporal stores of the output lattice
+    kFusedNtLoad = 2,       // non-temporal loads (only together with kFusedNtStore)
+    kFusedOcc4 = 4,         // cap registers for 4 waves per SIMD
+    kFusedIdentityMap = 8,  // no XCD-contiguous tile remapping
+};
 struct FusedCfg {
     int ty;      // tile rows (waves per block); tile is 64 x ty cells
-    int zchunk;  // planes marched per block
+    int zchunk;  // planes marched per block (0: whole box)
+    int flags;   // kFused* bits
 };
 
 template <class S>

@@ -93,15 +93,15 @@ int main(int argc, char **argv)
     CK(zsc_create(in.xmax, in.ymax, in.zmax, prec, nullptr, &ctx));
     CK(zsc_set_params(ctx, &p));
     hipEvent_t e0, e1, e2, e3;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
-    hipEventCreate(&e3);
-    hipEventRecord(e0, 0);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventCreate(&e2);
+    (void)hipEventCreate(&e3);
+    (void)hipEventRecord(e0, 0);
     CK(zsc_state_init_bubbles(ctx, bub.data(), nb));
     printf("Data initialization completed.\n");
-    hipDeviceSynchronize();
-    hipEventRecord(e1, 0);
+    (void)hipDeviceSynchronize();
+    (void)hipEventRecord(e1, 0);
 
     zsc_stats s;
     // relaxation: gravity off (src/main.cu:361-474)
@@ -129,7 +129,7 @@ int main(int argc, char **argv)
     }
     printf("Relaxation run completed.\n");
     CK(zsc_sync(ctx));
-    hipEventRecord(e2, 0);
+    (void)hipEventRecord(e2, 0);
 
     // evolution: gravity on (src/main.cu:479-610), maxStep-1 steps
     StatsFile evo{dir + "/stats.out"};
@@ -161,12 +161,12 @@ int main(int argc, char **argv)
     }
     printf("Evolution run completed.\n");
     CK(zsc_sync(ctx));
-    hipEventRecord(e3, 0);
-    hipEventSynchronize(e3);
+    (void)hipEventRecord(e3, 0);
+    (void)hipEventSynchronize(e3);
     float t_setup = 0, t_relax = 0, t_main = 0;
-    hipEventElapsedTime(&t_setup, e0, e1);
-    hipEventElapsedTime(&t_relax, e1, e2);
-    hipEventElapsedTime(&t_main, e2, e3);
+    (void)hipEventElapsedTime(&t_setup, e0, e1);
+    (void)hipEventElapsedTime(&t_relax, e1, e2);
+    (void)hipEventElapsedTime(&t_main, e2, e3);
     CK(zsc_state_stats(ctx, &s));
     const double cells = (double)in.xmax * in.ymax * in.zmax;
     const double mlups = last > 0 && evo_ms > 0 ? cells * last / (evo_ms * 1e3) : 0.0;

@@ -173,8 +173,8 @@ class Context:
     def set_gravity(self, on: bool):
         check(lib().zsc_set_gravity(self._h, int(bool(on))))
 
-    def set_fused_config(self, ty: int, zchunk: int):
-        check(lib().zsc_set_fused_config(self._h, ty, zchunk))
+    def set_fused_config(self, ty: int, zchunk: int, flags: int = 0):
+        check(lib().zsc_set_fused_config(self._h, ty, zchunk, flags))
 
     def sync(self):
         check(lib().zsc_sync(self._h))
