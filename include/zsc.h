@@ -224,6 +224,33 @@ int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
  * tuning knob of this implementation. */
 int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);
 
+/* ---- adjoint (no reference counterpart: the reference has no derivatives) */
+/* Hand-written reverse mode of the time step for F64 and F32 contexts.  All
+ * pointers are device pointers; lattices are plain pop-major (26, N) arrays
+ * of the context's element type (f_0..f_6 then g_0..g_18, population q at
+ * element q*N); phi arrays hold N elements; grads is a device double[16]
+ * that the calls accumulate into (index: 0 alpha4, 1 kappa, 2 inv_tau_phi_one,
+ * 3 inv_tau_phi, 4 inv_eta2, 5 eta, 6 inv_tau_rho_one, 7-9 k0..k2,
+ * 10-12 kc0..kc2, 13 grav, 14 the loss).  The gravity switch applies as in
+ * zsc_step.  A context's scratch buffer of zsc_adj_scratch_bytes() is passed
+ * in by the caller, which owns all memory. */
+size_t zsc_adj_scratch_bytes(zsc_ctx *ctx);
+/* a0 = init_f/init_g equilibria of phi0 (src/init_f.cu, src/init_g.cu). */
+int zsc_adj_init(zsc_ctx *ctx, const void *phi0, void *a0);
+/* a_out = one time step of a_in (update, collide, stream). */
+int zsc_adj_forward(zsc_ctx *ctx, const void *a_in, void *a_out, void *scratch);
+/* Loss L = sum (phi(a_n) - phi_tgt)^2 into grads[14]; lam = dL/da_n. */
+int zsc_adj_seed(zsc_ctx *ctx, const void *a_n, const void *phi_tgt, void *lam, void *scratch,
+                 double *grads);
+/* lam_out = J(a_n)^T lam_next for the step a_n -> a_{n+1}; grads += the
+ * step's coefficient derivatives. */
+int zsc_adj_backward(zsc_ctx *ctx, const void *a_n, const void *lam_next, void *lam_out,
+                     void *scratch, double *grads);
+/* grads += derivatives of the initial equilibria (alpha4, kappa) given
+ * lam0 = dL/da0. */
+int zsc_adj_init_backward(zsc_ctx *ctx, const void *phi0, const void *lam0, void *scratch,
+                          double *grads);
+
 /* Bytes per lattice update the fused step must move (26 populations read
  * and written once at the storage precision). */
 double zsc_bytes_per_cell(int precision);

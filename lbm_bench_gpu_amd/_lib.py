@@ -88,6 +88,12 @@ SIGNATURES = {
     "zsc_state_import": (_I, [_VP, _PP7, _PP19]),
     "zsc_state_stats": (_I, [_VP, ctypes.POINTER(ZscStats)]),
     "zsc_bytes_per_cell": (_D, [_I]),
+    "zsc_adj_scratch_bytes": (ctypes.c_size_t, [_VP]),
+    "zsc_adj_init": (_I, [_VP, _VP, _VP]),
+    "zsc_adj_forward": (_I, [_VP, _VP, _VP, _VP]),
+    "zsc_adj_seed": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "zsc_adj_backward": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "zsc_adj_init_backward": (_I, [_VP, _VP, _VP, _VP, _VP]),
 }
 
 _lib = None

@@ -1,0 +1,190 @@
+"""Differentiable ZSC runs: a ``torch.autograd.Function`` over libzsc's
+hand-written adjoint (include/zsc.h, zsc_adj_*).
+
+The differentiated program is
+
+    a_0 = init(phi0; theta)                  (init_f / init_g equilibria)
+    a_{k+1} = step(a_k; theta),  k < N       (update, collide, stream)
+    L = sum_x (phi(a_N)(x) - phi_tgt(x))^2
+
+with theta = (kappa, beta, tau_f, tau_g, gravity): beta is the reference's
+alpha (mu = 4 beta phi (phi^2 - phi*^2) - kappa lap phi), tau_f its tauPhi,
+tau_g its tauRho, and gravity enters as 2 phi* gravity on phi > 0 cells
+(src/paramread.c:141-170, src/collision_f.cu:96).
+
+The forward pass keeps a checkpoint every ``checkpoint_every`` steps; the
+backward pass recomputes each segment from its checkpoint, then runs the
+adjoint steps back through it.  All arithmetic runs in libzsc's HIP kernels;
+torch provides the device buffers.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import zsc
+from ._lib import check, lib
+
+THETA = ("kappa", "beta", "tau_f", "tau_g", "gravity")
+SLOTS = ("alpha4", "kappa", "inv_tau_phi_one", "inv_tau_phi", "inv_eta2", "eta",
+         "inv_tau_rho_one", "k0", "k1", "k2", "kc0", "kc1", "kc2", "grav", "loss")
+
+
+def theta_of(params: zsc.Params):
+    return [params.kappa, params.alpha, 1.0 / params.inv_tau_phi,
+            1.0 / (1.0 - params.inv_tau_rho_one), params.gravity]
+
+
+def coefficients(theta, base: zsc.Params) -> zsc.Params:
+    """Step coefficients for free (kappa, beta, tau_f, tau_g, gravity); the
+    other fields (phi*, Gamma, rho_mean, ...) come from ``base``.  Same
+    expressions as zsc_derive_params."""
+    kappa, beta, tau_f, tau_g, gravity = (float(t) for t in theta)
+    itr = 1.0 / tau_g
+    half = 1.0 - 0.5 * itr
+    eta = 1.0 / (tau_f + 0.5)
+    return base.replace(
+        alpha=beta, alpha4=beta * 4.0, kappa=kappa, eta=eta, eta2=1.0 - eta,
+        inv_eta2=0.5 / (1.0 - eta), inv_tau_phi=1.0 / tau_f, inv_tau_phi_one=1.0 - 1.0 / tau_f,
+        inv_tau_rho_one=1.0 - itr, k0=itr * (1.0 / 3.0), k1=itr * (1.0 / 18.0),
+        k2=itr * (1.0 / 36.0), kc0=half * 1.0, kc1=half * (1.0 / 6.0), kc2=half * (1.0 / 12.0),
+        gravity=gravity, grav=2.0 * base.phi_star * gravity)
+
+
+def chain_rule(g, theta, phi_star: float):
+    """dL/dtheta from dL/d(step coefficients) (the 16 accumulated slots)."""
+    _, _, tau_f, tau_g, _ = (float(t) for t in theta)
+    G = dict(zip(SLOTS, g))
+    eta = 1.0 / (tau_f + 0.5)
+    d_kappa = G["kappa"]
+    d_beta = 4.0 * G["alpha4"]
+    d_tau_f = ((G["inv_tau_phi_one"] - G["inv_tau_phi"]) / tau_f ** 2
+               - eta ** 2 * (G["eta"] + G["inv_eta2"] * 0.5 / (1.0 - eta) ** 2))
+    d_tau_g = ((G["inv_tau_rho_one"] - G["k0"] / 3.0 - G["k1"] / 18.0 - G["k2"] / 36.0) / tau_g ** 2
+               + (G["kc0"] + G["kc1"] / 6.0 + G["kc2"] / 12.0) * 0.5 / tau_g ** 2)
+    d_grav = 2.0 * phi_star * G["grav"]
+    return [d_kappa, d_beta, d_tau_f, d_tau_g, d_grav]
+
+
+def _vp(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class AdjointRunner:
+    """Owns a context and the device buffers of one differentiable run."""
+
+    def __init__(self, dims, base: zsc.Params, precision: str = "f32", checkpoint_every: int = 20,
+                 gravity_on: bool = True, device: str = "cuda"):
+        if precision not in ("f32", "f64"):
+            raise ValueError("the adjoint runs at f32 or f64")
+        self.dims = tuple(dims)
+        nx, ny, nz = self.dims
+        self.n = nx * ny * nz
+        self.base = base
+        self.k = int(checkpoint_every)
+        self.dtype = torch.float32 if precision == "f32" else torch.float64
+        self.device = device
+        self.ctx = zsc.Context(nx, ny, nz, precision=precision,
+                               stream=torch.cuda.current_stream().cuda_stream)
+        self.ctx.set_gravity(gravity_on)
+        nbytes = lib().zsc_adj_scratch_bytes(self.ctx._h)
+        self.scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        self.grads = torch.zeros(16, dtype=torch.float64, device=device)
+        self.checkpoints = {}
+        self.nsteps = 0
+        self.phi0 = None
+
+    def _lattice(self):
+        return torch.empty((26, self.n), dtype=self.dtype, device=self.device)
+
+    def _set_theta(self, theta):
+        self.theta = [float(t) for t in theta]
+        self.ctx.set_params(coefficients(self.theta, self.base))
+
+    def forward(self, theta, phi0: torch.Tensor, nsteps: int, keep=True) -> torch.Tensor:
+        """a_N; keeps checkpoints every k steps when ``keep``."""
+        self._set_theta(theta)
+        self.phi0 = phi0.to(self.device, self.dtype).contiguous().view(-1)
+        a, b = self._lattice(), self._lattice()
+        check(lib().zsc_adj_init(self.ctx._h, _vp(self.phi0), _vp(a)))
+        self.checkpoints = {0: a.clone()} if keep else {}
+        for s in range(nsteps):
+            check(lib().zsc_adj_forward(self.ctx._h, _vp(a), _vp(b), _vp(self.scratch)))
+            a, b = b, a
+            if keep and (s + 1) % self.k == 0 and s + 1 < nsteps:
+                self.checkpoints[s + 1] = a.clone()
+        self.nsteps = nsteps
+        self.final = a
+        return a
+
+    def loss(self, a_n: torch.Tensor, phi_tgt: torch.Tensor, lam: torch.Tensor | None = None):
+        self.grads.zero_()
+        tgt = phi_tgt.to(self.device, self.dtype).contiguous().view(-1)
+        lam = self._lattice() if lam is None else lam
+        check(lib().zsc_adj_seed(self.ctx._h, _vp(a_n), _vp(tgt), _vp(lam), _vp(self.scratch),
+                                 ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))))
+        return float(self.grads[14].item()), lam
+
+    def backward(self, lam: torch.Tensor, scale: float = 1.0):
+        """dL/dtheta by recomputing each checkpoint segment; lam = dL/da_N."""
+        gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
+        loss_slot = float(self.grads[14].item())
+        self.grads.zero_()
+        lam2 = self._lattice()
+        starts = sorted(self.checkpoints)
+        for i in range(len(starts) - 1, -1, -1):
+            s0 = starts[i]
+            s1 = starts[i + 1] if i + 1 < len(starts) else self.nsteps
+            states = [self.checkpoints[s0]]
+            for _ in range(s0, s1 - 1):
+                nxt = self._lattice()
+                check(lib().zsc_adj_forward(self.ctx._h, _vp(states[-1]), _vp(nxt),
+                                            _vp(self.scratch)))
+                states.append(nxt)
+            for st in reversed(states):
+                check(lib().zsc_adj_backward(self.ctx._h, _vp(st), _vp(lam), _vp(lam2),
+                                             _vp(self.scratch), gp))
+                lam, lam2 = lam2, lam
+        check(lib().zsc_adj_init_backward(self.ctx._h, _vp(self.phi0), _vp(lam),
+                                          _vp(self.scratch), gp))
+        g = self.grads.cpu().numpy()
+        g[14] = loss_slot
+        self.lam0 = lam
+        return [scale * x for x in chain_rule(g, self.theta, self.base.phi_star)]
+
+    def loss_and_grad(self, theta, phi0, phi_tgt, nsteps):
+        a = self.forward(theta, phi0, nsteps)
+        L, lam = self.loss(a, phi_tgt)
+        return L, self.backward(lam)
+
+    def loss_only(self, theta, phi0, phi_tgt, nsteps):
+        a = self.forward(theta, phi0, nsteps, keep=False)
+        return self.loss(a, phi_tgt)[0]
+
+
+class ZSCLoss(torch.autograd.Function):
+    """L(theta) = sum (phi_N - phi_tgt)^2 of a ZSC run; theta is a float64
+    tensor (kappa, beta, tau_f, tau_g, gravity)."""
+
+    @staticmethod
+    def forward(ctx, theta, phi0, phi_tgt, runner: AdjointRunner, nsteps: int):
+        a = runner.forward(theta.detach().cpu().tolist(), phi0, nsteps)
+        L, lam = runner.loss(a, phi_tgt)
+        ctx.runner = runner
+        ctx.lam = lam
+        return torch.tensor(L, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g = ctx.runner.backward(ctx.lam, scale=float(grad_out))
+        return torch.tensor(g, dtype=torch.float64), None, None, None, None
+
+
+def zsc_loss(theta: torch.Tensor, phi0, phi_tgt, runner: AdjointRunner, nsteps: int):
+    return ZSCLoss.apply(theta, phi0, phi_tgt, runner, nsteps)
+
+
+def theta_array(params: zsc.Params) -> np.ndarray:
+    return np.array(theta_of(params))

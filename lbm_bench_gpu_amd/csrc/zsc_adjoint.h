@@ -1,0 +1,419 @@
+// zsc_adjoint.h -- hand-written reverse mode of the ZSC time step (no
+// reference counterpart: the reference has no derivatives).
+//
+// Forward step (A-state form): A_{n+1} = S(C(A_n; theta)), with C the cell
+// collision of src/collision_f.cu / src/collision_g.cu, fed by the moments of
+// src/update_*.cu and the 19-point phi stencil, and S the streaming of
+// src/stream_f.cu / src/stream_g.cu.  The backward step maps
+// lam_{n+1} = dL/dA_{n+1} to lam_n = J(A_n)^T lam_{n+1}:
+//   1. S^T (a gather from downwind): lam_B,i(x) = lam_i(x + c_i) for g;
+//      eta lam_i(x + c_i) + eta2 lam_i(x) for f_1..6.
+//   2. C^T per cell: the transposed BGK + forcing Jacobian with respect to
+//      the local populations (through phi, rho, u) and to mu and grad(phi).
+//   3. The adjoint of mu = 4 beta phi (phi^2 - phi*^2) - kappa lap(phi):
+//      lap is symmetric, so its transpose is itself; the isotropic gradient
+//      is antisymmetric, so its transpose is minus itself.  A second pass
+//      adds L[a_lap] - div(a_grad) to every f population's adjoint.
+// The phi > 0 gravity switch (src/collision_f.cu:96) is a step function; its
+// derivative with respect to phi is taken as zero.
+// Alongside, dL/d(step coefficients) accumulates in FP64; the host maps those
+// to kappa, beta, tau_f, tau_g and gravity by the chain rule of the
+// parameter derivation (src/paramread.c:141-170).
+#pragma once
+
+#include "zsc_kernels.h"
+
+namespace zsc {
+
+// Coefficient slots of the parameter adjoint.
+enum : int {
+    kGAlpha4 = 0,
+    kGKappa,
+    kGInvTauPhiOne,
+    kGInvTauPhi,
+    kGInvEta2,
+    kGEta,
+    kGInvTauRhoOne,
+    kGK0,
+    kGK1,
+    kGK2,
+    kGKC0,
+    kGKC1,
+    kGKC2,
+    kGGrav,
+    kGLoss,  // the loss itself (seed kernel)
+    kNumGrad = 16
+};
+
+// Reverse mode of one cell's update + collision.  Inputs: the phi stencil p,
+// the post-stream populations fa/ga and the adjoints lf/lg of the
+// post-collision populations.  Outputs: the local adjoints afa/aga of fa/ga
+// (afa without the phi-stencil coupling), the adjoint of phi(x) through
+// its local uses, of lap(phi)(x) and of grad(phi)(x); fb receives the
+// forward post-collision f (for the eta derivative of the streaming).
+template <class R>
+__device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const R *ga,
+                                const R *lf, const R *lg, R *afa, R *aga, R &aphi, R &alap, R *agr,
+                                R *fb, double *ap, int grav_on)
+{
+    // ---- forward recompute
+    const R phi = p[0];
+    const R rho = sum_g(ga);
+    const R jx = ga[1] - ga[2] + ga[7] - ga[8] + ga[9] - ga[10] + ga[11] - ga[12] + ga[13] - ga[14];
+    const R jy = ga[3] - ga[4] + ga[7] - ga[8] - ga[9] + ga[10] + ga[15] - ga[16] + ga[17] - ga[18];
+    const R jz = ga[5] - ga[6] + ga[11] - ga[12] - ga[13] + ga[14] + ga[15] - ga[16] - ga[17] + ga[18];
+    const R u[3] = {jx / rho, jy / rho, jz / rho};
+    const R lap = lap19(p);
+    const R G[3] = {
+        (R(2) * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] + p[13] - p[14]) *
+            W<R>::inv12,
+        (R(2) * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] + p[17] - p[18]) *
+            W<R>::inv12,
+        (R(2) * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] + p[18] - p[17]) *
+            W<R>::inv12};
+    const R phi3 = phi * (phi * phi - c.phi_star_sq);
+    const R mu = c.alpha4 * phi3 - c.kappa * lap;
+    const R H = phi > R(0) ? R(1) : R(0);
+    R F[3] = {mu * G[0], mu * G[1], mu * G[2] + H * c.grav};
+    R v[3];
+    for (int a = 0; a < 3; ++a) v[a] = u[a] + R(0.5) * F[a] / rho;
+    const R hg = R(0.5) * c.gamma * c.inv_tau_phi;
+    const R cfk = c.inv_tau_phi * c.inv_eta2;
+    const R Af = hg * mu, Cf = cfk * phi;
+    fb[0] = c.inv_tau_phi_one * fa[0] - R(6) * Af + c.inv_tau_phi * phi;
+    for (int a = 0; a < 3; ++a) {
+        fb[1 + 2 * a] = c.inv_tau_phi_one * fa[1 + 2 * a] + Af + Cf * v[a];
+        fb[2 + 2 * a] = c.inv_tau_phi_one * fa[2 + 2 * a] + Af - Cf * v[a];
+    }
+
+    // ---- f collision (src/collision_f.cu:105-114), reversed
+    R a_phi = R(0), a_mu = R(0), a_rho = R(0), a_v[3] = {0, 0, 0}, a_F[3] = {0, 0, 0};
+    const R tf = c.inv_tau_phi_one;
+    R s_tf = R(0);
+    for (int i = 0; i < 7; ++i) {
+        afa[i] = tf * lf[i];
+        s_tf += fa[i] * lf[i];
+    }
+    ap[kGInvTauPhiOne] += (double)s_tf;
+    R a_Af = R(-6) * lf[0];
+    for (int i = 1; i < 7; ++i) a_Af += lf[i];
+    a_phi += c.inv_tau_phi * lf[0];
+    R a_Cf = R(0);
+    for (int a = 0; a < 3; ++a) {
+        const R d = lf[1 + 2 * a] - lf[2 + 2 * a];
+        a_Cf += v[a] * d;
+        a_v[a] += Cf * d;
+    }
+    a_mu += hg * a_Af;
+    a_phi += cfk * a_Cf;
+    ap[kGInvTauPhi] += (double)(phi * lf[0] + R(0.5) * c.gamma * mu * a_Af + c.inv_eta2 * phi * a_Cf);
+    ap[kGInvEta2] += (double)(c.inv_tau_phi * phi * a_Cf);
+
+    // ---- g collision (src/collision_g.cu:228-313), reversed
+    const R tg = c.inv_tau_rho_one;
+    R s_tg = R(0);
+    for (int i = 0; i < 19; ++i) {
+        aga[i] = tg * lg[i];
+        s_tg += ga[i] * lg[i];
+    }
+    ap[kGInvTauRhoOne] += (double)s_tg;
+    const R ut[3] = {R(3) * v[0], R(3) * v[1], R(3) * v[2]};
+    const R UF = v[0] * F[0] + v[1] * F[1] + v[2] * F[2];
+    const R Q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const R X = R(3) * phi * mu + rho * (R(1) - R(1.5) * Q);
+    R a_UF = R(0), a_Q = R(0), a_X = R(0), a_ut[3] = {0, 0, 0};
+    R g_k0, g_k1 = R(0), g_k2 = R(0), g_kc0, g_kc1 = R(0), g_kc2 = R(0);
+    {
+        const R D = lg[0];
+        g_k0 = (rho - R(6) * phi * mu - R(1.5) * rho * Q) * D;
+        a_rho += c.k0 * (R(1) - R(1.5) * Q) * D;
+        a_phi += R(-6) * c.k0 * mu * D;
+        a_mu += R(-6) * c.k0 * phi * D;
+        a_Q += R(-1.5) * c.k0 * rho * D;
+        g_kc0 = -UF * D;
+        a_UF += -c.kc0 * D;
+    }
+    for (int a = 0; a < 3; ++a) {
+        const R Sp = lg[1 + 2 * a] + lg[2 + 2 * a], Dl = lg[1 + 2 * a] - lg[2 + 2 * a];
+        const R w = ut[a];
+        a_X += c.k1 * Sp;
+        g_k1 += X * Sp + R(0.5) * rho * w * w * Sp + w * rho * Dl;
+        a_rho += R(0.5) * c.k1 * w * w * Sp + w * c.k1 * Dl;
+        a_ut[a] += (c.k1 * rho * w + c.kc1 * F[a]) * Sp + (c.k1 * rho + c.kc1 * F[a]) * Dl;
+        a_F[a] += c.kc1 * w * Sp + w * c.kc1 * Dl;
+        g_kc1 += (w * F[a] - UF) * Sp + w * F[a] * Dl;
+        a_UF += -c.kc1 * Sp;
+    }
+    const int da[6] = {0, 0, 0, 0, 1, 1}, db[6] = {1, 1, 2, 2, 2, 2};
+    const R sg[6] = {R(1), R(-1), R(1), R(-1), R(1), R(-1)};
+    for (int d = 0; d < 6; ++d) {
+        const int ip = 7 + 2 * d, im = 8 + 2 * d, a = da[d], b = db[d];
+        const R Sp = lg[ip] + lg[im], Dl = lg[ip] - lg[im];
+        const R s = ut[a] + sg[d] * ut[b], S = F[a] + sg[d] * F[b];
+        a_X += c.k2 * Sp;
+        g_k2 += X * Sp + R(0.5) * rho * s * s * Sp + rho * s * Dl;
+        a_rho += R(0.5) * c.k2 * s * s * Sp + c.k2 * s * Dl;
+        const R a_s = (c.k2 * rho * s + c.kc2 * S) * Sp + c.k2 * rho * Dl;
+        const R a_S = c.kc2 * s * Sp + c.kc2 * Dl;
+        g_kc2 += (s * S - UF) * Sp + S * Dl;
+        a_UF += -c.kc2 * Sp;
+        a_ut[a] += a_s;
+        a_ut[b] += sg[d] * a_s;
+        a_F[a] += a_S;
+        a_F[b] += sg[d] * a_S;
+    }
+    ap[kGK0] += (double)g_k0;
+    ap[kGK1] += (double)g_k1;
+    ap[kGK2] += (double)g_k2;
+    ap[kGKC0] += (double)g_kc0;
+    ap[kGKC1] += (double)g_kc1;
+    ap[kGKC2] += (double)g_kc2;
+    // X = 3 phi mu + rho (1 - 1.5 Q)
+    a_phi += R(3) * mu * a_X;
+    a_mu += R(3) * phi * a_X;
+    a_rho += (R(1) - R(1.5) * Q) * a_X;
+    a_Q += R(-1.5) * rho * a_X;
+    for (int a = 0; a < 3; ++a) {
+        a_v[a] += R(3) * a_ut[a] + R(2) * v[a] * a_Q + F[a] * a_UF;
+        a_F[a] += v[a] * a_UF;
+    }
+    // v = u + F / (2 rho)
+    R a_u[3];
+    for (int a = 0; a < 3; ++a) {
+        a_u[a] = a_v[a];
+        a_F[a] += a_v[a] * R(0.5) / rho;
+        a_rho += -F[a] * a_v[a] * R(0.5) / (rho * rho);
+    }
+    // F = mu grad(phi) + grav H(phi) e_z
+    a_mu += G[0] * a_F[0] + G[1] * a_F[1] + G[2] * a_F[2];
+    for (int a = 0; a < 3; ++a) agr[a] = mu * a_F[a];
+    if (grav_on) ap[kGGrav] += (double)(H * a_F[2]);
+    // mu = alpha4 phi (phi^2 - phi*^2) - kappa lap
+    a_phi += c.alpha4 * (R(3) * phi * phi - c.phi_star_sq) * a_mu;
+    ap[kGAlpha4] += (double)(phi3 * a_mu);
+    ap[kGKappa] += (double)(-lap * a_mu);
+    alap = -c.kappa * a_mu;
+    // u = j / rho, rho = sum g, j = sum c g
+    R a_j[3];
+    for (int a = 0; a < 3; ++a) {
+        a_j[a] = a_u[a] / rho;
+        a_rho += -u[a] * a_u[a] / rho;
+    }
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+        aga[i] += a_rho + R(cx(i)) * a_j[0] + R(cy(i)) * a_j[1] + R(cz(i)) * a_j[2];
+    aphi = a_phi;
+}
+
+// Wavefront + block reduction of kNumGrad doubles into partial[block].
+template <int NT>
+__device__ inline void block_reduce_grads(double *ap, double *partial)
+{
+    __shared__ double red[NT / 64][kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) {
+        double v = ap[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        ap[j] = v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int j = 0; j < kNumGrad; ++j) red[wv][j] = ap[j];
+    __syncthreads();
+    if (threadIdx.x < kNumGrad) {
+        double v = 0.0;
+        for (int w = 0; w < NT / 64; ++w) v += red[w][threadIdx.x];
+        partial[(int64_t)blockIdx.x * kNumGrad + threadIdx.x] = v;
+    }
+}
+
+constexpr int kAdjBlocks = 2048;
+
+// Pass 1: lam_out (local part), the three stencil-adjoint fields and
+// per-block coefficient partials.  A and lam are plain (26, n) lattices.
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
+                const R *__restrict__ phi, const R *__restrict__ lam, R *__restrict__ lam_out,
+                R *__restrict__ fields, double *partial)
+{
+    double ap[kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) ap[j] = 0.0;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        R p[19], fa[7], ga[19], lf[7], lg[19], afa[7], aga[19], fb[7], agr[3], aphi, alap;
+        gather19(phi, nb, p);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) fa[i] = A[(int64_t)i * n + idx];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) ga[i] = A[(int64_t)(7 + i) * n + idx];
+        // S^T: gather the next adjoint from downwind
+        R lfd[7];
+        lf[0] = lam[idx];
+        lfd[0] = R(0);
+#pragma unroll
+        for (int i = 1; i < 7; ++i) {
+            const R own = lam[(int64_t)i * n + idx];
+            lfd[i] = lam[(int64_t)i * n + nb.at(cx(i), cy(i), cz(i))];
+            lf[i] = c.eta * lfd[i] + c.eta2 * own;
+            lfd[i] -= own;  // for the eta derivative below
+        }
+#pragma unroll
+        for (int i = 0; i < 19; ++i) lg[i] = lam[(int64_t)(7 + i) * n + nb.at(cx(i), cy(i), cz(i))];
+        adj_cell(c, p, fa, ga, lf, lg, afa, aga, aphi, alap, agr, fb, ap, grav_on);
+        R deta = R(0);
+#pragma unroll
+        for (int i = 1; i < 7; ++i) deta += fb[i] * lfd[i];
+        ap[kGEta] += (double)deta;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) lam_out[(int64_t)i * n + idx] = afa[i];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) lam_out[(int64_t)(7 + i) * n + idx] = aga[i];
+        fields[idx] = aphi;
+        fields[n + idx] = alap;
+        fields[2 * n + idx] = agr[0];
+        fields[3 * n + idx] = agr[1];
+        fields[4 * n + idx] = agr[2];
+    }
+    block_reduce_grads<256>(ap, partial);
+}
+
+// Pass 2: a_phi = a_phi_local + lap[a_lap] - div[a_grad] (the transposed
+// stencils), added to the adjoint of every f population.
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_adj_pass2(int nx, int ny, int nz, int64_t n, const R *__restrict__ fields,
+                R *__restrict__ lam_out)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        R h[19];
+        gather19(fields + n, nb, h);
+        R a = fields[idx] + lap19(h);
+        gather19(fields + 2 * n, nb, h);
+        R dv = R(2) * (h[1] - h[2]) + h[7] - h[8] + h[9] - h[10] + h[11] - h[12] + h[13] - h[14];
+        gather19(fields + 3 * n, nb, h);
+        dv += R(2) * (h[3] - h[4]) + h[7] - h[8] + h[10] - h[9] + h[15] - h[16] + h[17] - h[18];
+        gather19(fields + 4 * n, nb, h);
+        dv += R(2) * (h[5] - h[6]) + h[11] - h[12] + h[14] - h[13] + h[15] - h[16] + h[18] - h[17];
+        a -= dv * W<R>::inv12;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) lam_out[(int64_t)i * n + idx] += a;
+    }
+}
+
+// Loss seed: L = sum (phi - phi_tgt)^2, lam_f,i = 2 (phi - phi_tgt), lam_g = 0.
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_adj_seed(int64_t n, const R *__restrict__ A, const R *__restrict__ tgt, R *__restrict__ lam,
+               double *partial)
+{
+    double ap[kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) ap[j] = 0.0;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        R f[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) f[i] = A[(int64_t)i * n + idx];
+        const R d = sum_f(f) - tgt[idx];
+        ap[kGLoss] += (double)d * (double)d;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) lam[(int64_t)i * n + idx] = R(2) * d;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) lam[(int64_t)(7 + i) * n + idx] = R(0);
+    }
+    block_reduce_grads<256>(ap, partial);
+}
+
+// Adjoint of the initial equilibria (src/init_f.cu:69-78, src/init_g.cu:74-91)
+// with respect to alpha4 and kappa, at fixed phi0 and rho = rho_mean.
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_adj_init(int nx, int ny, int nz, int64_t n, Coef<R> c, const R *__restrict__ phi0,
+               const R *__restrict__ lam, double *partial)
+{
+    double ap[kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) ap[j] = 0.0;
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        Nbr nb(x, y, z, nx, ny, nz);
+        R p[19];
+        gather19(phi0, nb, p);
+        const R phi = p[0];
+        R a_mu = R(-3) * c.gamma * lam[idx];
+#pragma unroll
+        for (int i = 1; i < 7; ++i) a_mu += R(0.5) * c.gamma * lam[(int64_t)i * n + idx];
+        a_mu += R(-6) * W<R>::w0 * phi * lam[7 * n + idx];
+#pragma unroll
+        for (int i = 1; i < 7; ++i) a_mu += R(3) * W<R>::w1 * phi * lam[(int64_t)(7 + i) * n + idx];
+#pragma unroll
+        for (int i = 7; i < 19; ++i) a_mu += R(3) * W<R>::w2 * phi * lam[(int64_t)(7 + i) * n + idx];
+        ap[kGAlpha4] += (double)(phi * (phi * phi - c.phi_star_sq) * a_mu);
+        ap[kGKappa] += (double)(-lap19(p) * a_mu);
+    }
+    block_reduce_grads<256>(ap, partial);
+}
+
+// grads[j] += sum over blocks of partial[b][j] (fixed order).
+template <class R>
+__global__ void __launch_bounds__(64) k_adj_accumulate(int nblocks, const double *partial, double *grads)
+{
+    const int j = threadIdx.x;
+    if (j >= kNumGrad) return;
+    double v = 0.0;
+    for (int b = 0; b < nblocks; ++b) v += partial[(int64_t)b * kNumGrad + j];
+    grads[j] += v;
+}
+
+template <class R>
+struct Adj {
+    // scratch: 5n R (stencil fields) + kAdjBlocks*kNumGrad doubles (partials)
+    static size_t scratch_bytes(int64_t n)
+    {
+        return align256((size_t)n * 5 * sizeof(R)) + (size_t)kAdjBlocks * kNumGrad * sizeof(double);
+    }
+    static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
+                               const R *A, const R *phi, const R *lam, R *lam_out, void *scratch,
+                               double *grads)
+    {
+        R *fields = (R *)scratch;
+        double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
+        hipLaunchKernelGGL(k_adj_pass1<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                           make_coef<R>(p, grav_on), grav_on, A, phi, lam, lam_out, fields, partial);
+        hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                           (const R *)fields, lam_out);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(64), 0, s, kAdjBlocks, partial, grads);
+        return hipGetLastError();
+    }
+    static hipError_t seed(const Geo &g, hipStream_t s, const R *A, const R *tgt, R *lam,
+                           void *scratch, double *grads)
+    {
+        double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
+        hipLaunchKernelGGL(k_adj_seed<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.n, A, tgt, lam, partial);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(64), 0, s, kAdjBlocks, partial, grads);
+        return hipGetLastError();
+    }
+    static hipError_t init_backward(const Geo &g, const zsc_params &p, hipStream_t s, const R *phi0,
+                                    const R *lam, void *scratch, double *grads)
+    {
+        double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
+        hipLaunchKernelGGL(k_adj_init<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                           make_coef<R>(p, 0), phi0, lam, partial);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(64), 0, s, kAdjBlocks, partial, grads);
+        return hipGetLastError();
+    }
+};
+
+}  // namespace zsc

@@ -11,7 +11,7 @@
 
 #include "../../include/zsc.h"
 #include "zsc_err.h"
-#include "zsc_launch.h"
+#include "zsc_adjoint.h"
 
 namespace zsc {
 
@@ -611,4 +611,93 @@ extern "C" int zsc_state_export(zsc_ctx *c, void *const f[7], void *const g[19])
 extern "C" int zsc_state_import(zsc_ctx *c, void *const f[7], void *const g[19])
 {
     return xport<false>(c, f, g, "zsc_state_import");
+}
+
+// ---------------------------------------------------------------------------
+// adjoint (F64 / F32 contexts; plain (26, N) lattices, device pointers)
+// ---------------------------------------------------------------------------
+extern "C" size_t zsc_adj_scratch_bytes(zsc_ctx *c)
+{
+    if (!c) return 0;
+    const size_t n = (size_t)c->geo.n;
+    // forward: B lattice (26n) + phi (n); backward: phi (n) + Adj scratch
+    const size_t fwd = align256(26 * n * c->rsize) + align256(n * c->rsize);
+    const size_t bwd = align256(n * c->rsize) +
+                       (c->prec == ZSC_F64 ? Adj<double>::scratch_bytes(c->geo.n)
+                                           : Adj<float>::scratch_bytes(c->geo.n));
+    return fwd > bwd ? fwd : bwd;
+}
+
+template <class F>
+static int adj_call(zsc_ctx *c, const char *name, F &&f)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
+    return dropin(c, false, name, std::forward<F>(f));
+}
+
+extern "C" int zsc_adj_init(zsc_ctx *c, const void *phi0, void *a0)
+{
+    if (!phi0 || !a0) return fail(ZSC_ERR_ARG, "zsc_adj_init: null pointer");
+    return adj_call(c, "zsc_adj_init", [&](auto tag) {
+        using T = ZSC_T;
+        return Ops<T>::lat_init(c->geo, c->params, c->stream, phi0, nullptr, a0);
+    });
+}
+
+extern "C" int zsc_adj_forward(zsc_ctx *c, const void *a_in, void *a_out, void *scratch)
+{
+    if (!a_in || !a_out || !scratch || a_in == a_out)
+        return fail(ZSC_ERR_ARG, "zsc_adj_forward: null or aliased pointer");
+    return adj_call(c, "zsc_adj_forward", [&](auto tag) {
+        using T = ZSC_T;
+        const size_t n = (size_t)c->geo.n;
+        void *B = scratch;
+        void *phi = (char *)scratch + align256(26 * n * sizeof(T));
+        hipError_t e = Ops<T>::lat_phi(c->geo, c->stream, a_in, phi);
+        if (e == hipSuccess)
+            e = Ops<T>::lat_collide(c->geo, c->params, c->grav_on, c->stream, a_in, phi, B);
+        if (e == hipSuccess) e = Ops<T>::lat_stream(c->geo, c->params, c->stream, B, a_out);
+        return e;
+    });
+}
+
+extern "C" int zsc_adj_seed(zsc_ctx *c, const void *a_n, const void *phi_tgt, void *lam,
+                            void *scratch, double *grads)
+{
+    if (!a_n || !phi_tgt || !lam || !scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_seed: null pointer");
+    return adj_call(c, "zsc_adj_seed", [&](auto tag) {
+        using T = ZSC_T;
+        void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
+        return Adj<T>::seed(c->geo, c->stream, (const T *)a_n, (const T *)phi_tgt, (T *)lam, adj_scr,
+                            grads);
+    });
+}
+
+extern "C" int zsc_adj_backward(zsc_ctx *c, const void *a_n, const void *lam_next, void *lam_out,
+                                void *scratch, double *grads)
+{
+    if (!a_n || !lam_next || !lam_out || !scratch || !grads || lam_next == lam_out)
+        return fail(ZSC_ERR_ARG, "zsc_adj_backward: null or aliased pointer");
+    return adj_call(c, "zsc_adj_backward", [&](auto tag) {
+        using T = ZSC_T;
+        void *phi = scratch;
+        void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
+        hipError_t e = Ops<T>::lat_phi(c->geo, c->stream, a_n, phi);
+        if (e != hipSuccess) return e;
+        return Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)a_n,
+                                (const T *)phi, (const T *)lam_next, (T *)lam_out, adj_scr, grads);
+    });
+}
+
+extern "C" int zsc_adj_init_backward(zsc_ctx *c, const void *phi0, const void *lam0, void *scratch,
+                                     double *grads)
+{
+    if (!phi0 || !lam0 || !scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_init_backward: null pointer");
+    return adj_call(c, "zsc_adj_init_backward", [&](auto tag) {
+        using T = ZSC_T;
+        void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
+        return Adj<T>::init_backward(c->geo, c->params, c->stream, (const T *)phi0, (const T *)lam0,
+                                     adj_scr, grads);
+    });
 }

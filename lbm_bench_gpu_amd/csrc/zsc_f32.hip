@@ -1,10 +1,11 @@
 // zsc_f32.hip -- FP32-compute instantiations of libzsc's kernels: FP32
 // storage, and FP32 compute over bf16 / fp16 deviation storage.
-#include "zsc_kernels.h"
+#include "zsc_adjoint.h"
 
 namespace zsc {
 template struct Ops<float>;
 template struct Ops<bf16s>;
 template struct Ops<f16s>;
 template struct DropIn<float>;
+template struct Adj<float>;
 }  // namespace zsc

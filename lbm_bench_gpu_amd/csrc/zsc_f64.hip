@@ -1,9 +1,10 @@
 // zsc_f64.hip -- FP64 instantiation of libzsc's kernels (parity mode).
 // Built with -ffp-contract=off: every multiply and add is rounded on its own,
 // as in an IEEE evaluation of the reference's expressions.
-#include "zsc_kernels.h"
+#include "zsc_adjoint.h"
 
 namespace zsc {
 template struct Ops<double>;
 template struct DropIn<double>;
+template struct Adj<double>;
 }  // namespace zsc

@@ -18,14 +18,15 @@ struct Geo {
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Byte layout of one context lattice: f_0 | f_1..6 | g_0..18, each block
-// 256-byte aligned.
+// Byte layout of one context lattice: f_0 | f_1..6 | g_0..18, packed.  For
+// FP64 and FP32 this is exactly a pop-major (26, n) array, population q at
+// element q*n, which the adjoint entry points exchange with callers.
 template <class S>
 struct LatLayout {
     using S0 = typename Traits<S>::S0;
-    static size_t f_off(int64_t n) { return align256((size_t)n * sizeof(S0)); }
-    static size_t g_off(int64_t n) { return f_off(n) + align256((size_t)n * 6 * sizeof(S)); }
-    static size_t bytes(int64_t n) { return g_off(n) + align256((size_t)n * 19 * sizeof(S)); }
+    static size_t f_off(int64_t n) { return (size_t)n * sizeof(S0); }
+    static size_t g_off(int64_t n) { return f_off(n) + (size_t)n * 6 * sizeof(S); }
+    static size_t bytes(int64_t n) { return g_off(n) + (size_t)n * 19 * sizeof(S); }
     static Lattice<S> view(void *base, int64_t n)
     {
         char *b = (char *)base;
@@ -111,5 +112,9 @@ struct DropIn {
     static hipError_t import_state(const Geo &, hipStream_t, void *A, T *const f[7],
                                    T *const g[19]);
 };
+
+// Reverse mode of the step on plain (26, n) lattices (zsc_adjoint.h).
+template <class R>
+struct Adj;
 
 }  // namespace zsc

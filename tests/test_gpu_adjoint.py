@@ -1,0 +1,117 @@
+"""Hand-written adjoint (zsc_adj_*) against independent derivatives:
+  * per step: <J v, w> = <v, J^T w> with J v from central differences (FP64);
+  * whole run: FP64 gradients against a taped FP64 CPU gradient (torch
+    autograd over oracle/taped.py) on a 32^3 box;
+  * the 128^3 adjoint workload: gradients against central finite differences
+    (h = 1e-4 relative) of FP64 runs.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbm_bench_gpu_amd import adjoint, configs, zsc
+from lbm_bench_gpu_amd._lib import check, lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def small_problem(dims, seed=0):
+    nx, ny, nz = dims
+    p = zsc.derive_params(configs.physics(*dims, gravity=1e-3), 4.0)
+    rng = np.random.default_rng(seed)
+    b = np.array([[nx / 2 + 0.3, ny / 2 - 0.2, nz / 2 + 1.1, 4.0], [2.0, 3.0, 2.5, 3.0]])
+    phi0 = O.init_phi((nz, ny, nx), b, p.phi_star, p.width)
+    phi0 = phi0 + 0.02 * rng.standard_normal(phi0.shape)
+    return p, phi0
+
+
+def test_step_dot_product(need_gpu):
+    import torch
+    dims = (12, 10, 8)
+    p, phi0 = small_problem(dims)
+    run = adjoint.AdjointRunner(dims, p, precision="f64", checkpoint_every=4)
+    a = run.forward(adjoint.theta_of(p), torch.from_numpy(phi0), 3)
+    n = run.n
+    g = torch.Generator(device="cuda").manual_seed(1)
+    v = torch.randn((26, n), dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn((26, n), dtype=torch.float64, device="cuda", generator=g)
+    eps = 1e-6
+    ap, am = a + eps * v, a - eps * v
+    op, om = run._lattice(), run._lattice()
+    check(lib().zsc_adj_forward(run.ctx._h, _vp(ap), _vp(op), _vp(run.scratch)))
+    check(lib().zsc_adj_forward(run.ctx._h, _vp(am), _vp(om), _vp(run.scratch)))
+    jv = (op - om) / (2 * eps)
+    lam = run._lattice()
+    gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
+    check(lib().zsc_adj_backward(run.ctx._h, _vp(a), _vp(w), _vp(lam), _vp(run.scratch), gp))
+    lhs = float((jv * w).sum())
+    rhs = float((v * lam).sum())
+    print("dot product", lhs, rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-7 * abs(rhs)
+
+
+def test_fp64_gradients_match_taped_cpu(need_gpu):
+    import torch
+    from oracle import taped
+    dims = (32, 32, 32)
+    nx, ny, nz = dims
+    p = zsc.derive_params(configs.physics(*dims), 6.0)
+    b = np.array([[16.3, 15.6, 16.9, 6.0]])
+    phi0 = O.init_phi((nz, ny, nx), b, p.phi_star, p.width)
+    steps = 30
+    theta = adjoint.theta_of(p)
+    # target from perturbed physics, as the 128^3 workload
+    tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
+    run = adjoint.AdjointRunner(dims, p, precision="f64", checkpoint_every=7)
+    tgt = run.forward(tt, torch.from_numpy(phi0), steps, keep=False)[:7].sum(0).clone()
+    L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, steps)
+    # taped FP64 CPU reference (init inside the tape)
+    th = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in theta]
+    c = taped.coefficients(*th, phi_star=p.phi_star, gamma=p.gamma)
+    f, g = taped.init_state(torch.from_numpy(phi0), c, p.rho_mean)
+    for _ in range(steps):
+        f, g = taped.step(f, g, c)
+    tgt_cpu = tgt.view(nz, ny, nx).cpu()
+    loss = ((f.sum(0) - tgt_cpu) ** 2).sum()
+    ref = [float(x) for x in torch.autograd.grad(loss, th)]
+    print("loss", L, float(loss), "\nadjoint", grads, "\ntaped  ", ref, flush=True)
+    assert abs(L - float(loss)) <= 1e-10 * abs(float(loss))
+    for a, r in zip(grads, ref):
+        assert abs(a - r) <= 1e-8 * max(abs(r), 1e-30), (a, r)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_adjoint_workload_vs_central_differences(need_gpu, precision):
+    """128^3, 8 bubbles R = 12 (seed 4), 400 steps, checkpoints every 20;
+    target from kappa x1.2, beta x0.9, gravity x1.5; FD h = 1e-4 relative on
+    FP64 runs."""
+    import torch
+    w = configs.workload(4)
+    p = w.params()
+    nx, ny, nz = w.dims
+    phi0 = torch.from_numpy(zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width))
+    theta = adjoint.theta_of(p)
+    tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
+    fd_run = adjoint.AdjointRunner(w.dims, p, precision="f64", checkpoint_every=20)
+    tgt = fd_run.forward(tt, phi0, w.steps, keep=False)[:7].sum(0).clone()
+    run = adjoint.AdjointRunner(w.dims, p, precision=precision, checkpoint_every=20)
+    L, grads = run.loss_and_grad(theta, phi0, tgt, w.steps)
+    fd = []
+    for i in range(5):
+        h = 1e-4 * abs(theta[i])
+        tp = list(theta)
+        tm = list(theta)
+        tp[i] += h
+        tm[i] -= h
+        fd.append((fd_run.loss_only(tp, phi0, tgt, w.steps) - fd_run.loss_only(tm, phi0, tgt, w.steps))
+                  / (2 * h))
+    print(precision, "loss", L, "\nadjoint", grads, "\nFD     ", fd, flush=True)
+    tol = 1e-4 if precision == "f64" else 2e-3
+    for name, a, r in zip(adjoint.THETA, grads, fd):
+        assert abs(a - r) <= tol * abs(r), (name, a, r)
