@@ -86,8 +86,11 @@ class AdjointRunner:
         self.k = int(checkpoint_every)
         self.dtype = torch.float32 if precision == "f32" else torch.float64
         self.device = device
-        self.ctx = zsc.Context(nx, ny, nz, precision=precision,
-                               stream=torch.cuda.current_stream().cuda_stream)
+        # A stream of our own (default priority): the library's kernels and
+        # the torch ops on these buffers are ordered on it.  torch's default
+        # stream has handle 0, which the C-ABI reads as "make a new stream".
+        self.stream = torch.cuda.Stream(device=device)
+        self.ctx = zsc.Context(nx, ny, nz, precision=precision, stream=self.stream.cuda_stream)
         self.ctx.set_gravity(gravity_on)
         nbytes = lib().zsc_adj_scratch_bytes(self.ctx._h)
         self.scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
@@ -105,6 +108,13 @@ class AdjointRunner:
 
     def forward(self, theta, phi0: torch.Tensor, nsteps: int, keep=True) -> torch.Tensor:
         """a_N; keeps checkpoints every k steps when ``keep``."""
+        torch.cuda.current_stream().synchronize()
+        with torch.cuda.stream(self.stream):
+            a = self._forward(theta, phi0, nsteps, keep)
+        self.stream.synchronize()
+        return a
+
+    def _forward(self, theta, phi0, nsteps, keep):
         self._set_theta(theta)
         self.phi0 = phi0.to(self.device, self.dtype).contiguous().view(-1)
         a, b = self._lattice(), self._lattice()
@@ -120,6 +130,13 @@ class AdjointRunner:
         return a
 
     def loss(self, a_n: torch.Tensor, phi_tgt: torch.Tensor, lam: torch.Tensor | None = None):
+        torch.cuda.current_stream().synchronize()
+        with torch.cuda.stream(self.stream):
+            out = self._loss(a_n, phi_tgt, lam)
+        self.stream.synchronize()
+        return out
+
+    def _loss(self, a_n, phi_tgt, lam):
         self.grads.zero_()
         tgt = phi_tgt.to(self.device, self.dtype).contiguous().view(-1)
         lam = self._lattice() if lam is None else lam
@@ -129,6 +146,13 @@ class AdjointRunner:
 
     def backward(self, lam: torch.Tensor, scale: float = 1.0):
         """dL/dtheta by recomputing each checkpoint segment; lam = dL/da_N."""
+        torch.cuda.current_stream().synchronize()
+        with torch.cuda.stream(self.stream):
+            g = self._backward(lam, scale)
+        self.stream.synchronize()
+        return g
+
+    def _backward(self, lam, scale):
         gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
         loss_slot = float(self.grads[14].item())
         self.grads.zero_()

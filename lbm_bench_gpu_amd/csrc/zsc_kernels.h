@@ -197,6 +197,93 @@ __device__ inline void pull_g(const Lattice<S> &B, int64_t n, const Nbr &nb, con
         g[i] = ldg<S, R, FL>(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
 }
 
+// ---- 32-bit-offset access for the fused kernel.  Every population array
+// of a lattice is at most 4 GiB (checked at launch), so a byte offset within
+// it fits 32 bits; with a uniform per-population base the loads and stores
+// take the scalar-base + 32-bit vector-offset form.
+struct Nbr32 {
+    uint32_t cols[3], rows[3], planes[3];
+    __device__ Nbr32(int x, int y, int z, int nx, int ny, int nz)
+    {
+        const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
+        cols[0] = (uint32_t)wrap_dec(x, nx);
+        cols[1] = (uint32_t)x;
+        cols[2] = (uint32_t)wrap_inc(x, nx);
+        rows[0] = (uint32_t)wrap_dec(y, ny) * (uint32_t)nx;
+        rows[1] = (uint32_t)y * (uint32_t)nx;
+        rows[2] = (uint32_t)wrap_inc(y, ny) * (uint32_t)nx;
+        planes[0] = (uint32_t)wrap_dec(z, nz) * nxy;
+        planes[1] = (uint32_t)z * nxy;
+        planes[2] = (uint32_t)wrap_inc(z, nz) * nxy;
+    }
+    __device__ uint32_t at(int dx, int dy, int dz) const
+    {
+        return cols[1 + dx] + rows[1 + dy] + planes[1 + dz];
+    }
+};
+
+template <int FL, class S, class R>
+__device__ inline R ld32(const S *base, uint32_t i, R rest)
+{
+    const S *p = (const S *)((const char *)base + i * (uint32_t)sizeof(S));
+    return Codec<S>::dec(ld_raw<FL>(p), rest);
+}
+
+template <int FL, class S, class R>
+__device__ inline void st32(S *base, uint32_t i, R v, R rest)
+{
+    S *p = (S *)((char *)base + i * (uint32_t)sizeof(S));
+    st_raw<FL>(p, Codec<S>::enc(v, rest));
+}
+
+// zero rest value
+template <int FL, class S>
+__device__ inline auto ld32z(const S *base, uint32_t i)
+{
+    const S *p = (const S *)((const char *)base + i * (uint32_t)sizeof(S));
+    return Codec<S>::dec0(ld_raw<FL>(p));
+}
+
+template <int FL, class S, class R>
+__device__ inline void st32z(S *base, uint32_t i, R v)
+{
+    S *p = (S *)((char *)base + i * (uint32_t)sizeof(S));
+    st_raw<FL>(p, Codec<S>::enc0(v));
+}
+
+template <class S, class R, int FL>
+__device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const Nbr32 &nb, const Coef<R> &c,
+                                R *f)
+{
+    const uint32_t own = nb.at(0, 0, 0);
+    f[0] = ld32z<FL>(B.f0, own);
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+        const S *p = B.f + (int64_t)(i - 1) * n;
+        f[i] = c.eta * ld32z<FL>(p, nb.at(-cx(i), -cy(i), -cz(i))) + c.eta2 * ld32z<FL>(p, own);
+    }
+}
+
+template <class S, class R, int FL>
+__device__ inline void pull_g32(const Lattice<S> &B, int64_t n, const Nbr32 &nb, const Coef<R> &c,
+                                R *g)
+{
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+        g[i] = ld32<FL>(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
+}
+
+template <class S, class R, int FL>
+__device__ inline void store32(const Lattice<S> &L, int64_t n, uint32_t idx, const Coef<R> &c,
+                               const R *f, const R *g)
+{
+    st32z<FL>(L.f0, idx, f[0]);
+#pragma unroll
+    for (int i = 1; i < 7; ++i) st32z<FL>(L.f + (int64_t)(i - 1) * n, idx, f[i]);
+#pragma unroll
+    for (int i = 0; i < 19; ++i) st32<FL>(L.g + (int64_t)i * n, idx, g[i], g_rest(c, i));
+}
+
 template <class S>
 __global__ void __launch_bounds__(256)
     k_lat_init(int nx, int ny, int nz, int64_t n, Coef<typename Traits<S>::R> c,
@@ -402,8 +489,8 @@ __global__ void __launch_bounds__(64 * TY, MINW)
         const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
         const int slot = (zz + 4) & 3;
         if (own) {
-            Nbr nb(x, y, zc, nx, ny, nz);
-            pull_f<S, R, FL>(in, n, nb, c, fown);
+            Nbr32 nb(x, y, zc, nx, ny, nz);
+            pull_f32<S, R, FL>(in, n, nb, c, fown);
             sp[slot][ty + 1][tx + 1] = sum_f(fown);
         }
         for (int h = tid; h < nh; h += 64 * TY) {
@@ -431,9 +518,9 @@ __global__ void __launch_bounds__(64 * TY, MINW)
             }
             hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
             hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
-            Nbr nb(hx, hy, zc, nx, ny, nz);
+            Nbr32 nb(hx, hy, zc, nx, ny, nz);
             R f[7];
-            pull_f<S, R, FL>(in, n, nb, c, f);
+            pull_f32<S, R, FL>(in, n, nb, c, f);
             sp[slot][r][col] = sum_f(f);
         }
     };
@@ -443,8 +530,8 @@ __global__ void __launch_bounds__(64 * TY, MINW)
     plane_phi(z0, fcur);
     for (int k = z0; k < z1; ++k) {
         R ga[19];
-        Nbr nb(x, y, k, nx, ny, nz);
-        if (own) pull_g<S, R, FL>(in, n, nb, c, ga);
+        Nbr32 nb(x, y, k, nx, ny, nz);
+        if (own) pull_g32<S, R, FL>(in, n, nb, c, ga);
         plane_phi(k + 1, fnext);
         __syncthreads();
         if (own) {
@@ -472,7 +559,7 @@ __global__ void __launch_bounds__(64 * TY, MINW)
             p[18] = sp[spl][r - 1][q];
             R fb[7], gb[19];
             collide_cell(c, p, fcur, ga, fb, gb);
-            store_local<S, R, FL>(out, n, nb.at(0, 0, 0), c, fb, gb);
+            store32<S, R, FL>(out, n, nb.at(0, 0, 0), c, fb, gb);
         }
 #pragma unroll
         for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
@@ -613,6 +700,10 @@ static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R>
     const int nbz = (g.nz + zchunk - 1) / zchunk;
     const int64_t nblk = (int64_t)nbx * nby * nbz;
     if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+    // 32-bit byte offsets within each population array
+    if ((uint64_t)g.n * sizeof(typename Traits<S>::S0) > ((uint64_t)1 << 32) ||
+        (uint64_t)g.n * sizeof(S) > ((uint64_t)1 << 32))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_fused<S, TY, FL, MINW>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
                        g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk,
                        (cfg.flags & kFusedIdentityMap) ? 0 : 1, c,

@@ -22,11 +22,15 @@ struct f16s {
 template <class S>
 struct Codec;
 
+// dec/enc take the population's rest value; dec0/enc0 are the same for a
+// rest value of exactly zero (no "+ 0.0", which IEEE arithmetic cannot drop).
 template <>
 struct Codec<double> {
     static constexpr bool deviation = false;
     __host__ __device__ static double dec(double s, double) { return s; }
     __host__ __device__ static double enc(double v, double) { return v; }
+    __host__ __device__ static double dec0(double s) { return s; }
+    __host__ __device__ static double enc0(double v) { return v; }
 };
 
 template <>
@@ -34,12 +38,18 @@ struct Codec<float> {
     static constexpr bool deviation = false;
     __host__ __device__ static float dec(float s, float) { return s; }
     __host__ __device__ static float enc(float v, float) { return v; }
+    __host__ __device__ static float dec0(float s) { return s; }
+    __host__ __device__ static float enc0(float v) { return v; }
 };
 
 // bf16: upper half of the FP32 pattern, rounded to nearest-even; NaN kept
-// quiet so divergence still shows in the reductions.
+// quiet so divergence still shows in the reductions.  gfx950 converts in one
+// instruction (v_cvt_pk_bf16_f32, round-to-nearest-even).
 __host__ __device__ inline uint16_t f32_to_bf16_rne(float f)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+#endif
     uint32_t u;
     __builtin_memcpy(&u, &f, 4);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
@@ -59,6 +69,8 @@ struct Codec<bf16s> {
     static constexpr bool deviation = true;
     __host__ __device__ static float dec(bf16s s, float rest) { return bf16_to_f32(s.v) + rest; }
     __host__ __device__ static bf16s enc(float v, float rest) { return {f32_to_bf16_rne(v - rest)}; }
+    __host__ __device__ static float dec0(bf16s s) { return bf16_to_f32(s.v); }
+    __host__ __device__ static bf16s enc0(float v) { return {f32_to_bf16_rne(v)}; }
 };
 
 template <>
@@ -70,9 +82,16 @@ struct Codec<f16s> {
         __builtin_memcpy(&h, &s.v, 2);
         return (float)h + rest;
     }
-    __host__ __device__ static f16s enc(float v, float rest)
+    __host__ __device__ static f16s enc(float v, float rest) { return enc0(v - rest); }
+    __host__ __device__ static float dec0(f16s s)
     {
-        _Float16 h = (_Float16)(v - rest);
+        _Float16 h;
+        __builtin_memcpy(&h, &s.v, 2);
+        return (float)h;
+    }
+    __host__ __device__ static f16s enc0(float v)
+    {
+        _Float16 h = (_Float16)v;
         f16s s;
         __builtin_memcpy(&s.v, &h, 2);
         return s;

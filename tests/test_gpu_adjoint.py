@@ -44,12 +44,14 @@ def test_step_dot_product(need_gpu):
     eps = 1e-6
     ap, am = a + eps * v, a - eps * v
     op, om = run._lattice(), run._lattice()
+    lam = run._lattice()
+    torch.cuda.synchronize()
     check(lib().zsc_adj_forward(run.ctx._h, _vp(ap), _vp(op), _vp(run.scratch)))
     check(lib().zsc_adj_forward(run.ctx._h, _vp(am), _vp(om), _vp(run.scratch)))
-    jv = (op - om) / (2 * eps)
-    lam = run._lattice()
     gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
     check(lib().zsc_adj_backward(run.ctx._h, _vp(a), _vp(w), _vp(lam), _vp(run.scratch), gp))
+    run.ctx.sync()
+    jv = (op - om) / (2 * eps)
     lhs = float((jv * w).sum())
     rhs = float((v * lam).sum())
     print("dot product", lhs, rhs, flush=True)
