@@ -38,19 +38,29 @@ def parse():
                     help="extra storage precisions timed beside fp32")
     ap.add_argument("--ty", type=int, default=0, help="fused tile rows (0: library default)")
     ap.add_argument("--zchunk", type=int, default=-1, help="fused z chunk (-1: library default)")
+    ap.add_argument("--flags", type=int, default=-1, help="fused flags (-1: library default)")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the FP32/bf16/fp16 vs FP64 accuracy runs (0 disables)")
+    ap.add_argument("--adjoint-steps", type=int, default=400,
+                    help="steps of the 128^3 adjoint timing (0 disables)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline (0 disables)")
     return ap.parse_args()
 
 
-def time_precision(prec, w, params, steps, warmup, ty, zchunk):
+def configure(ctx, ty, zchunk, flags):
+    if ty > 0 or zchunk >= 0 or flags >= 0:
+        ctx.set_fused_config(ty if ty > 0 else 4, zchunk if zchunk >= 0 else 16,
+                             flags if flags >= 0 else 1)
+
+
+def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags):
     import torch
     from lbm_bench_gpu_amd import zsc
     nx, ny, nz = w.dims
     ctx = zsc.Context(nx, ny, nz, precision=prec)
     try:
-        if ty > 0 or zchunk >= 0:
-            ctx.set_fused_config(ty if ty > 0 else 4, max(zchunk, 0) if zchunk >= 0 else 64)
+        configure(ctx, ty, zchunk, flags)
         ctx.set_params(params)
         ctx.state_init_bubbles(w.bubbles)
         ctx.step(warmup + 1)  # the first step also runs the A-lattice prologue
@@ -77,6 +87,83 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk):
         "sum_phi": st["sum_phi"],
         "finite": bool(np.isfinite(st["sum_phi"])),
     }
+
+
+def accuracy(w, params, steps, precs, ty, zchunk, flags):
+    """max and RMS error of phi, rho, |u| after `steps` steps of each precision
+    against an FP64 run of the same box on the same GPU, and the drift of
+    total phi (FP64 device reductions)."""
+    from lbm_bench_gpu_amd import zsc
+
+    def run(prec):
+        ctx = zsc.Context(*w.dims, precision=prec)
+        try:
+            if prec != "f64":
+                configure(ctx, ty, zchunk, flags)
+            ctx.set_params(params)
+            ctx.state_init_bubbles(w.bubbles)
+            s0 = ctx.stats()["sum_phi"]
+            ctx.step(steps)
+            s1 = ctx.stats()["sum_phi"]
+            phi, rho, ux, uy, uz = ctx.moments()
+        finally:
+            ctx.close()
+        umag = np.sqrt(ux * ux + uy * uy + uz * uz)
+        del ux, uy, uz
+        return phi, rho, umag, s0, s1
+
+    ref = run("f64")
+    out = {"steps": steps, "f64_sum_phi_drift": ref[4] - ref[3]}
+    for prec in precs:
+        print(f"# accuracy {prec}", file=sys.stderr, flush=True)
+        r = run(prec)
+        e = {}
+        for name, a, b in zip(("phi", "rho", "u"), r[:3], ref[:3]):
+            d = a - b
+            e[f"{name}_max"] = float(np.abs(d).max())
+            e[f"{name}_rms"] = float(np.sqrt(np.mean(d * d)))
+            del d
+        e["sum_phi_drift"] = r[4] - r[3]
+        e["sum_phi_drift_rel"] = (r[4] - r[3]) / abs(r[3])
+        out[prec] = e
+        del r
+    return out
+
+
+def adjoint_timing(steps: int, fwd_mlups_fp32: float):
+    """MLUPS of the checkpointed adjoint on the 128^3, 8-bubble workload
+    (FP32, checkpoints every 20 steps): cells * steps / time of the backward
+    pass, which includes the recomputation of every segment."""
+    import torch
+    from lbm_bench_gpu_amd import adjoint, configs, zsc
+    w = configs.workload(4)
+    p = w.params()
+    phi0 = torch.from_numpy(zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width))
+    theta = adjoint.theta_of(p)
+    tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
+    run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20)
+    tgt = run.forward(tt, phi0, steps, keep=False)[:7].sum(0).clone()
+    t0 = time.perf_counter()
+    a = run.forward(theta, phi0, steps)
+    t1 = time.perf_counter()
+    L, lam = run.loss(a, tgt)
+    t2 = time.perf_counter()
+    grads = run.backward(lam)
+    t3 = time.perf_counter()
+    # forward MLUPS of the fused FP32 step on the same box
+    ctx = zsc.Context(*w.dims, precision="f32")
+    ctx.set_params(p)
+    ctx.state_init_bubbles(w.bubbles)
+    ctx.step(5)
+    ms = ctx.step_timed(steps)
+    ctx.close()
+    fused = w.cells * steps / (ms / 1e3) / 1e6
+    adj = w.cells * steps / (t3 - t2) / 1e6
+    return {"workload": w.name, "steps": steps, "checkpoint_every": 20, "precision": "f32",
+            "loss": L, "grad": dict(zip(adjoint.THETA, grads)),
+            "forward_with_checkpoints_mlups": round(w.cells * steps / (t1 - t0) / 1e6, 1),
+            "adjoint_mlups": round(adj, 1), "fused_forward_mlups": round(fused, 1),
+            "adjoint_vs_forward": round(adj / fused, 3)}
 
 
 def cpu_baseline(seconds: float):
@@ -126,11 +213,19 @@ def main():
                      a.steps, ("f32",))
     params = w.params()
     print(f"# workload {w.name}: fp32 {a.steps} steps", file=sys.stderr, flush=True)
-    main_r = time_precision("f32", w, params, a.steps, a.warmup, a.ty, a.zchunk)
+    main_r = time_precision("f32", w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags)
     extra = {}
-    for v in [s for s in a.variants.split(",") if s]:
+    variants = [s for s in a.variants.split(",") if s]
+    for v in variants:
         print(f"# {v} storage", file=sys.stderr, flush=True)
-        extra[v] = time_precision(v, w, params, a.steps, a.warmup, a.ty, a.zchunk)
+        extra[v] = time_precision(v, w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags)
+    acc = None
+    if a.accuracy_steps > 0:
+        acc = accuracy(w, params, a.accuracy_steps, ["f32"] + variants, a.ty, a.zchunk, a.flags)
+    adj = None
+    if a.adjoint_steps > 0:
+        print("# adjoint", file=sys.stderr, flush=True)
+        adj = adjoint_timing(a.adjoint_steps, main_r["mlups"])
     cpu = cpu_baseline(a.cpu_seconds) if a.cpu_seconds > 0 else None
     out = {
         "metric": "MLUPS + %HBM roofline, FP32 vs bf16-store (err vs FP64); adjoint MLUPS",
@@ -152,6 +247,8 @@ def main():
         "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                          "speedup_vs_fp32": round(v["mlups"] / main_r["mlups"], 3),
                          "roofline": v["roofline"]} for k, v in extra.items()},
+        "accuracy_vs_fp64": acc,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

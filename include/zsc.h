@@ -219,8 +219,10 @@ int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
 /* Launch shape of the fused kernel: ty wavefront rows of 64 cells per block
  * (1, 2, 4 or 8), zchunk planes marched per block (0: the whole box) and
  * flags: 1 non-temporal output stores, 2 (with 1) non-temporal loads, 4 cap
- * registers for 4 waves per SIMD, 8 no XCD-contiguous tile order.  Flags
- * only apply to the FP32-compute precisions.  No reference counterpart; a
+ * registers for 4 waves per SIMD, 8 no XCD-contiguous tile order, and bits
+ * 8..15 a number of KiB (<= 128) of unused LDS per block that limits how
+ * many blocks share a CU.  Bits 0-2 only apply to the FP32-compute
+ * precisions.  No reference counterpart; a
  * tuning knob of this implementation. */
 int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);
 

@@ -117,6 +117,13 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     c->geo = {nx, ny, nz, n};
     c->prec = precision;
     c->rsize = precision == ZSC_F64 ? 8 : 4;
+    // fused-kernel launch shape per precision, from 512^3 sweeps on MI355X
+    if (precision == ZSC_F32)
+        c->cfg = {8, 16, kFusedNtStore};
+    else if (precision == ZSC_BF16 || precision == ZSC_FP16)
+        c->cfg = {4, 32, kFusedNtStore};
+    else
+        c->cfg = {4, 16, 0};
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) {
         if (stream) {
@@ -170,8 +177,8 @@ extern "C" int zsc_set_gravity(zsc_ctx *c, int on)
 extern "C" int zsc_set_fused_config(zsc_ctx *c, int ty, int zchunk, int flags)
 {
     if (int r = need_ctx(c)) return r;
-    if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || zchunk < 0 || flags < 0 || flags > 15 ||
-        (flags & 3) == kFusedNtLoad)
+    if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || zchunk < 0 || flags < 0 ||
+        (flags & 0xf0) != 0 || (flags >> 8) > 128 || (flags & 3) == kFusedNtLoad)
         return fail(ZSC_ERR_ARG, "zsc_set_fused_config: ty in {1,2,4,8}, zchunk >= 0, valid flags");
     c->cfg = {ty, zchunk, flags};
     return ZSC_OK;

@@ -704,7 +704,9 @@ static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R>
     if ((uint64_t)g.n * sizeof(typename Traits<S>::S0) > ((uint64_t)1 << 32) ||
         (uint64_t)g.n * sizeof(S) > ((uint64_t)1 << 32))
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_fused<S, TY, FL, MINW>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
+    // optional dynamic LDS that only limits how many blocks share a CU
+    const unsigned pad = (unsigned)((cfg.flags >> 8) & 0xff) * 1024u;
+    hipLaunchKernelGGL((k_fused<S, TY, FL, MINW>), dim3((unsigned)nblk), dim3(64, TY), pad, s,
                        g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk,
                        (cfg.flags & kFusedIdentityMap) ? 0 : 1, c,
                        LatLayout<S>::view(const_cast<void *>(Bin), g.n),
