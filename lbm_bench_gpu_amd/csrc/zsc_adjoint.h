@@ -367,14 +367,33 @@ __global__ void __launch_bounds__(256)
 }
 
 // grads[j] += sum over blocks of partial[b][j] (fixed order).
+// grads[j] += sum over blocks of partial[b][j]: 256 threads, strided sums in
+// a fixed order, then a fixed-shape shuffle tree (deterministic).
 template <class R>
-__global__ void __launch_bounds__(64) k_adj_accumulate(int nblocks, const double *partial, double *grads)
+__global__ void __launch_bounds__(256) k_adj_accumulate(int nblocks, const double *partial, double *grads)
 {
-    const int j = threadIdx.x;
-    if (j >= kNumGrad) return;
-    double v = 0.0;
-    for (int b = 0; b < nblocks; ++b) v += partial[(int64_t)b * kNumGrad + j];
-    grads[j] += v;
+    double ap[kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) ap[j] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+        for (int j = 0; j < kNumGrad; ++j) ap[j] += partial[(int64_t)b * kNumGrad + j];
+    __shared__ double red[4][kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) {
+        double v = ap[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        ap[j] = v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int j = 0; j < kNumGrad; ++j) red[wv][j] = ap[j];
+    __syncthreads();
+    if (threadIdx.x < kNumGrad) {
+        const int j = threadIdx.x;
+        grads[j] += (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+    }
 }
 
 template <class R>
@@ -394,7 +413,7 @@ struct Adj {
                            make_coef<R>(p, grav_on), grav_on, A, phi, lam, lam_out, fields, partial);
         hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            (const R *)fields, lam_out);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(64), 0, s, kAdjBlocks, partial, grads);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
         return hipGetLastError();
     }
     static hipError_t seed(const Geo &g, hipStream_t s, const R *A, const R *tgt, R *lam,
@@ -402,7 +421,7 @@ struct Adj {
     {
         double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
         hipLaunchKernelGGL(k_adj_seed<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.n, A, tgt, lam, partial);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(64), 0, s, kAdjBlocks, partial, grads);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
         return hipGetLastError();
     }
     static hipError_t init_backward(const Geo &g, const zsc_params &p, hipStream_t s, const R *phi0,
@@ -411,7 +430,7 @@ struct Adj {
         double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
         hipLaunchKernelGGL(k_adj_init<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            make_coef<R>(p, 0), phi0, lam, partial);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(64), 0, s, kAdjBlocks, partial, grads);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
         return hipGetLastError();
     }
 };

@@ -38,6 +38,26 @@ inline Coef<R> make_coef(const zsc_params &p, int grav_on)
     c.kc2 = (R)p.kc2;
     c.grav = grav_on ? (R)p.grav : R(0);
     c.rho_mean = (R)p.rho_mean;
+    if constexpr (sizeof(R) == 4) {
+        // Rounding each FP32 coefficient on its own leaves t + 1/tau,
+        // eta + eta2 and t_g + K0 + 6 K1 + 12 K2 a few ulps off one: a
+        // systematic per-step drift of sum(phi) and sum(rho).  Form them so
+        // the sums are exact: 1 - a is exact for a in [1/2, 2] (Sterbenz);
+        // K2 keeps 22 significant bits so that 24 K2 is exact, and
+        // K0 = 1/tau - 24 K2 is exact again.  (FP64 keeps the host values.)
+        c.inv_tau_phi_one = R(1) - c.inv_tau_phi;
+        c.eta2 = R(1) - c.eta;
+        const float a = (float)(1.0 - p.inv_tau_rho_one);
+        c.inv_tau_rho_one = 1.0f - a;
+        float k2 = (float)p.k2;
+        uint32_t u;
+        __builtin_memcpy(&u, &k2, 4);
+        u = (u + 2u) & ~3u;  // round to 22 significant bits
+        __builtin_memcpy(&k2, &u, 4);
+        c.k2 = k2;
+        c.k1 = 2.0f * k2;
+        c.k0 = a - 24.0f * k2;
+    }
     return c;
 }
 
@@ -131,12 +151,12 @@ __device__ inline void st_raw(S *p, S v)
     }
 }
 
-template <class S, class R, int FL = 0>
+template <int FL = 0, class S, class R>
 __device__ inline R ldg(const S *p, int64_t i, R rest)
 {
     return Codec<S>::dec(ld_raw<FL>(p + i), rest);
 }
-template <class S, class R, int FL = 0>
+template <int FL = 0, class S, class R>
 __device__ inline void stg(S *p, int64_t i, R v, R rest)
 {
     st_raw<FL>(p + i, Codec<S>::enc(v, rest));
@@ -164,11 +184,11 @@ template <class S, class R, int FL = 0>
 __device__ inline void store_local(const Lattice<S> &L, int64_t n, int64_t idx, const Coef<R> &c,
                                    const R *f, const R *g)
 {
-    stg<typename Traits<S>::S0, R, FL>(L.f0, idx, f[0], R(0));
+    stg<FL>(L.f0, idx, f[0], R(0));
 #pragma unroll
-    for (int i = 1; i < 7; ++i) stg<S, R, FL>(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
+    for (int i = 1; i < 7; ++i) stg<FL>(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
 #pragma unroll
-    for (int i = 0; i < 19; ++i) stg<S, R, FL>(L.g, (int64_t)i * n + idx, g[i], g_rest(c, i));
+    for (int i = 0; i < 19; ++i) stg<FL>(L.g, (int64_t)i * n + idx, g[i], g_rest(c, i));
 }
 
 // Post-stream f of one cell, pulled from a B lattice (src/stream_f.cu:52-59
@@ -178,12 +198,11 @@ __device__ inline void pull_f(const Lattice<S> &B, int64_t n, const Nbr &nb, con
                               R *f)
 {
     const int64_t own = nb.at(0, 0, 0);
-    f[0] = ldg<typename Traits<S>::S0, R, FL>(B.f0, own, R(0));
+    f[0] = ldg<FL>(B.f0, own, R(0));
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
-        const S *p = B.f + (int64_t)(i - 1) * n;
-        f[i] = c.eta * ldg<S, R, FL>(p, nb.at(-cx(i), -cy(i), -cz(i)), R(0)) +
-               c.eta2 * ldg<S, R, FL>(p, own, R(0));
+        const auto *p = B.f + (int64_t)(i - 1) * n;
+        f[i] = c.eta * ldg<FL>(p, nb.at(-cx(i), -cy(i), -cz(i)), R(0)) + c.eta2 * ldg<FL>(p, own, R(0));
     }
 }
 
@@ -194,7 +213,7 @@ __device__ inline void pull_g(const Lattice<S> &B, int64_t n, const Nbr &nb, con
 {
 #pragma unroll
     for (int i = 0; i < 19; ++i)
-        g[i] = ldg<S, R, FL>(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
+        g[i] = ldg<FL>(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
 }
 
 // ---- 32-bit-offset access for the fused kernel.  Every population array
@@ -259,7 +278,7 @@ __device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const Nbr32 &nb,
     f[0] = ld32z<FL>(B.f0, own);
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
-        const S *p = B.f + (int64_t)(i - 1) * n;
+        const auto *p = B.f + (int64_t)(i - 1) * n;
         f[i] = c.eta * ld32z<FL>(p, nb.at(-cx(i), -cy(i), -cz(i))) + c.eta2 * ld32z<FL>(p, own);
     }
 }
@@ -380,24 +399,22 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-template <class S>
+template <class S0, class T, class R>
 __global__ void __launch_bounds__(256)
-    k_pop_get(int64_t n, const typename Traits<S>::S0 *__restrict__ f0,
-              const S *__restrict__ p, int is_f0, typename Traits<S>::R rest, double *dst)
+    k_pop_get(int64_t n, const S0 *__restrict__ f0, const T *__restrict__ p, int is_f0, R rest,
+              double *dst)
 {
-    using R = typename Traits<S>::R;
     ZSC_GRID_STRIDE(idx, n)
     {
         dst[idx] = is_f0 ? (double)ldg(f0, idx, R(0)) : (double)ldg(p, idx, rest);
     }
 }
 
-template <class S>
+template <class S0, class T, class R>
 __global__ void __launch_bounds__(256)
-    k_pop_set(int64_t n, typename Traits<S>::S0 *__restrict__ f0, S *__restrict__ p, int is_f0,
-              typename Traits<S>::R rest, const double *src)
+    k_pop_set(int64_t n, S0 *__restrict__ f0, T *__restrict__ p, int is_f0, R rest,
+              const double *src)
 {
-    using R = typename Traits<S>::R;
     ZSC_GRID_STRIDE(idx, n)
     {
         if (is_f0)
@@ -637,16 +654,35 @@ __global__ void __launch_bounds__(256)
 }
 
 template <class R>
-__global__ void __launch_bounds__(64) k_stats_final(int nblocks, const double *partial, double *out)
+__global__ void __launch_bounds__(256) k_stats_final(int nblocks, const double *partial, double *out)
 {
-    const int j = threadIdx.x;
-    if (j >= 10) return;
-    double v = 0.0;
-    for (int b = 0; b < nblocks; ++b) {
-        const double p = partial[(int64_t)b * 10 + j];
-        v = (j == 6) ? (p > v ? p : v) : v + p;
+    // strided partial sums in a fixed order, then a fixed shuffle tree
+    double a[10];
+    for (int j = 0; j < 10; ++j) a[j] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+        for (int j = 0; j < 10; ++j) {
+            const double p = partial[(int64_t)b * 10 + j];
+            a[j] = (j == 6) ? (p > a[j] ? p : a[j]) : a[j] + p;
+        }
+    __shared__ double red[4][10];
+    for (int j = 0; j < 10; ++j) {
+        double v = a[j];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v, off, 64);
+            v = (j == 6) ? (o > v ? o : v) : v + o;
+        }
+        a[j] = v;
     }
-    out[j] = v;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int j = 0; j < 10; ++j) red[wv][j] = a[j];
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        const int j = threadIdx.x;
+        double v = red[0][j];
+        for (int w = 1; w < 4; ++w) v = (j == 6) ? (red[w][j] > v ? red[w][j] : v) : v + red[w][j];
+        out[j] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -702,7 +738,8 @@ static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R>
     if (nblk > 0x7fffffff) return hipErrorInvalidValue;
     // 32-bit byte offsets within each population array
     if ((uint64_t)g.n * sizeof(typename Traits<S>::S0) > ((uint64_t)1 << 32) ||
-        (uint64_t)g.n * sizeof(S) > ((uint64_t)1 << 32))
+        (uint64_t)g.n * sizeof(typename Traits<S>::SF) > ((uint64_t)1 << 32) ||
+        (uint64_t)g.n * sizeof(typename Traits<S>::SG) > ((uint64_t)1 << 32))
         return hipErrorInvalidValue;
     // optional dynamic LDS that only limits how many blocks share a CU
     const unsigned pad = (unsigned)((cfg.flags >> 8) & 0xff) * 1024u;
@@ -765,10 +802,16 @@ hipError_t Ops<S>::pop_get(const Geo &g, const zsc_params &p, hipStream_t s, con
 {
     const Lattice<S> v = LatLayout<S>::view(const_cast<void *>(L), g.n);
     const Coef<R> c = make_coef<R>(p, 0);
-    const S *ptr = q == 0 ? nullptr : (q < 7 ? v.f + (int64_t)(q - 1) * g.n : v.g + (int64_t)(q - 7) * g.n);
-    const R rest = q < 7 ? R(0) : g_weight<R>(q - 7) * c.rho_mean;
-    hipLaunchKernelGGL(k_pop_get<S>, grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, q == 0 ? 1 : 0,
-                       rest, dst);
+    if (q < 7) {
+        const auto *ptr = q == 0 ? v.f : v.f + (int64_t)(q - 1) * g.n;
+        hipLaunchKernelGGL((k_pop_get<typename Traits<S>::S0, typename Traits<S>::SF, R>),
+                           grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, q == 0 ? 1 : 0, R(0), dst);
+    } else {
+        const auto *ptr = v.g + (int64_t)(q - 7) * g.n;
+        hipLaunchKernelGGL((k_pop_get<typename Traits<S>::S0, typename Traits<S>::SG, R>),
+                           grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, 0,
+                           g_weight<R>(q - 7) * c.rho_mean, dst);
+    }
     return hipGetLastError();
 }
 
@@ -778,10 +821,16 @@ hipError_t Ops<S>::pop_set(const Geo &g, const zsc_params &p, hipStream_t s, voi
 {
     const Lattice<S> v = LatLayout<S>::view(L, g.n);
     const Coef<R> c = make_coef<R>(p, 0);
-    S *ptr = q == 0 ? nullptr : (q < 7 ? v.f + (int64_t)(q - 1) * g.n : v.g + (int64_t)(q - 7) * g.n);
-    const R rest = q < 7 ? R(0) : g_weight<R>(q - 7) * c.rho_mean;
-    hipLaunchKernelGGL(k_pop_set<S>, grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, q == 0 ? 1 : 0,
-                       rest, src);
+    if (q < 7) {
+        auto *ptr = q == 0 ? v.f : v.f + (int64_t)(q - 1) * g.n;
+        hipLaunchKernelGGL((k_pop_set<typename Traits<S>::S0, typename Traits<S>::SF, R>),
+                           grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, q == 0 ? 1 : 0, R(0), src);
+    } else {
+        auto *ptr = v.g + (int64_t)(q - 7) * g.n;
+        hipLaunchKernelGGL((k_pop_set<typename Traits<S>::S0, typename Traits<S>::SG, R>),
+                           grid_for(g.n), dim3(256), 0, s, g.n, v.f0, ptr, 0,
+                           g_weight<R>(q - 7) * c.rho_mean, src);
+    }
     return hipGetLastError();
 }
 
@@ -818,7 +867,7 @@ hipError_t Ops<S>::stats(const Geo &g, const zsc_params &p, hipStream_t s, const
                        (const R *)ux, (const R *)uy, (const R *)uz, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_stats_final<R>, dim3(1), dim3(64), 0, s, kStatsBlocks, partial, out);
+    hipLaunchKernelGGL(k_stats_final<R>, dim3(1), dim3(256), 0, s, kStatsBlocks, partial, out);
     return hipGetLastError();
 }
 

@@ -24,13 +24,15 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 template <class S>
 struct LatLayout {
     using S0 = typename Traits<S>::S0;
+    using SF = typename Traits<S>::SF;
+    using SG = typename Traits<S>::SG;
     static size_t f_off(int64_t n) { return (size_t)n * sizeof(S0); }
-    static size_t g_off(int64_t n) { return f_off(n) + (size_t)n * 6 * sizeof(S); }
-    static size_t bytes(int64_t n) { return g_off(n) + (size_t)n * 19 * sizeof(S); }
+    static size_t g_off(int64_t n) { return f_off(n) + (size_t)n * 6 * sizeof(SF); }
+    static size_t bytes(int64_t n) { return g_off(n) + (size_t)n * 19 * sizeof(SG); }
     static Lattice<S> view(void *base, int64_t n)
     {
         char *b = (char *)base;
-        return {(S0 *)b, (S *)(b + f_off(n)), (S *)(b + g_off(n))};
+        return {(S0 *)b, (SF *)(b + f_off(n)), (SG *)(b + g_off(n))};
     }
 };
 

@@ -98,20 +98,23 @@ struct Codec<f16s> {
     }
 };
 
-// Compute type and f_0 storage type for a population storage type S.
+// Compute type R and the storage types of f_0 (S0), f_1..6 (SF) and g (SG)
+// for a precision tag S.  bf16 mode keeps the f populations, which carry
+// phi, in fp16 (four times finer than bf16 at the same 2 bytes) and g in
+// bf16; f_0 ~ phi is O(1) and stays FP32 in both 16-bit modes.
 template <class S> struct Traits;
-template <> struct Traits<double> { using R = double; using S0 = double; };
-template <> struct Traits<float> { using R = float; using S0 = float; };
-template <> struct Traits<bf16s> { using R = float; using S0 = float; };
-template <> struct Traits<f16s> { using R = float; using S0 = float; };
+template <> struct Traits<double> { using R = double; using S0 = double; using SF = double; using SG = double; };
+template <> struct Traits<float> { using R = float; using S0 = float; using SF = float; using SG = float; };
+template <> struct Traits<bf16s> { using R = float; using S0 = float; using SF = f16s; using SG = bf16s; };
+template <> struct Traits<f16s> { using R = float; using S0 = float; using SF = f16s; using SG = f16s; };
 
 // One lattice (A or B) in the context's layout: f_0 array, f_1..6 with
 // stride n, g_0..18 with stride n.  Pointers are device addresses.
 template <class S>
 struct Lattice {
     typename Traits<S>::S0 *f0;
-    S *f;  // f_i at f + (i-1)*n, i = 1..6
-    S *g;  // g_i at g + i*n
+    typename Traits<S>::SF *f;  // f_i at f + (i-1)*n, i = 1..6
+    typename Traits<S>::SG *g;  // g_i at g + i*n
 };
 
 }  // namespace zsc
