@@ -244,10 +244,21 @@ int zsc_adj_forward(zsc_ctx *ctx, const void *a_in, void *a_out, void *scratch);
 /* Loss L = sum (phi(a_n) - phi_tgt)^2 into grads[14]; lam = dL/da_n. */
 int zsc_adj_seed(zsc_ctx *ctx, const void *a_n, const void *phi_tgt, void *lam, void *scratch,
                  double *grads);
+/* b = collision of a (post-collision lattice), phi = phi(a). */
+int zsc_adj_collide(zsc_ctx *ctx, const void *a, void *b, void *phi);
+/* b_out = one fused step from the post-collision lattice b_in (stream, then
+ * update + collide); phi (may be NULL) receives phi of the post-stream state
+ * in between. */
+int zsc_adj_fused(zsc_ctx *ctx, const void *b_in, void *b_out, void *phi);
+/* a = streaming of the post-collision lattice b. */
+int zsc_adj_stream(zsc_ctx *ctx, const void *b, void *a);
 /* lam_out = J(a_n)^T lam_next for the step a_n -> a_{n+1}; grads += the
- * step's coefficient derivatives. */
-int zsc_adj_backward(zsc_ctx *ctx, const void *a_n, const void *lam_next, void *lam_out,
-                     void *scratch, double *grads);
+ * step's coefficient derivatives.  state is a_n itself (pulled = 0; phi_n may
+ * be NULL and is then computed) or the previous post-collision lattice b_{n-1}
+ * from which a_n is streamed on the fly (pulled = 1; phi_n = phi(a_n)
+ * required). */
+int zsc_adj_backward(zsc_ctx *ctx, const void *state, int pulled, const void *phi_n,
+                     const void *lam_next, void *lam_out, void *scratch, double *grads);
 /* grads += derivatives of the initial equilibria (alpha4, kappa) given
  * lam0 = dL/da0. */
 int zsc_adj_init_backward(zsc_ctx *ctx, const void *phi0, const void *lam0, void *scratch,

@@ -114,18 +114,32 @@ class AdjointRunner:
         self.stream.synchronize()
         return a
 
+    def _phi(self):
+        return torch.empty(self.n, dtype=self.dtype, device=self.device)
+
     def _forward(self, theta, phi0, nsteps, keep):
+        # Steps run as fused launches on post-collision lattices:
+        # B_0 = collide(A_0), B_n = fused(B_{n-1}), A_N = stream(B_{N-1}).
+        # The checkpoint of step s is A_0 (s = 0) or B_{s-1}.
         self._set_theta(theta)
         self.phi0 = phi0.to(self.device, self.dtype).contiguous().view(-1)
-        a, b = self._lattice(), self._lattice()
-        check(lib().zsc_adj_init(self.ctx._h, _vp(self.phi0), _vp(a)))
-        self.checkpoints = {0: a.clone()} if keep else {}
-        for s in range(nsteps):
-            check(lib().zsc_adj_forward(self.ctx._h, _vp(a), _vp(b), _vp(self.scratch)))
-            a, b = b, a
-            if keep and (s + 1) % self.k == 0 and s + 1 < nsteps:
-                self.checkpoints[s + 1] = a.clone()
+        h = self.ctx._h
+        a0 = self._lattice()
+        check(lib().zsc_adj_init(h, _vp(self.phi0), _vp(a0)))
+        self.checkpoints = {0: a0} if keep else {}
         self.nsteps = nsteps
+        if nsteps == 0:
+            self.final = a0
+            return a0
+        b, b2, phi = self._lattice(), self._lattice(), self._phi()
+        check(lib().zsc_adj_collide(h, _vp(a0), _vp(b), _vp(phi)))
+        for s in range(1, nsteps):
+            if keep and s % self.k == 0:
+                self.checkpoints[s] = b.clone()
+            check(lib().zsc_adj_fused(h, _vp(b), _vp(b2), None))
+            b, b2 = b2, b
+        a = b2
+        check(lib().zsc_adj_stream(h, _vp(b), _vp(a)))
         self.final = a
         return a
 
@@ -153,6 +167,7 @@ class AdjointRunner:
         return g
 
     def _backward(self, lam, scale):
+        h = self.ctx._h
         gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
         loss_slot = float(self.grads[14].item())
         self.grads.zero_()
@@ -161,18 +176,30 @@ class AdjointRunner:
         for i in range(len(starts) - 1, -1, -1):
             s0 = starts[i]
             s1 = starts[i + 1] if i + 1 < len(starts) else self.nsteps
-            states = [self.checkpoints[s0]]
-            for _ in range(s0, s1 - 1):
-                nxt = self._lattice()
-                check(lib().zsc_adj_forward(self.ctx._h, _vp(states[-1]), _vp(nxt),
-                                            _vp(self.scratch)))
-                states.append(nxt)
-            for st in reversed(states):
-                check(lib().zsc_adj_backward(self.ctx._h, _vp(st), _vp(lam), _vp(lam2),
-                                             _vp(self.scratch), gp))
+            # recompute the segment: the input state of every step and phi_n
+            states, phis = [], []
+            if s0 == 0:
+                b, phi = self._lattice(), self._phi()
+                check(lib().zsc_adj_collide(h, _vp(self.checkpoints[0]), _vp(b), _vp(phi)))
+                states.append(self.checkpoints[0])
+                phis.append(phi)
+                n0 = 1
+            else:
+                b = self.checkpoints[s0]
+                n0 = s0
+            for _ in range(n0, s1):
+                nb, phi = self._lattice(), self._phi()
+                check(lib().zsc_adj_fused(h, _vp(b), _vp(nb), _vp(phi)))
+                states.append(b)
+                phis.append(phi)
+                b = nb
+            for n in range(s1 - 1, s0 - 1, -1):
+                j = n - s0
+                check(lib().zsc_adj_backward(h, _vp(states[j]), 0 if n == 0 else 1, _vp(phis[j]),
+                                             _vp(lam), _vp(lam2), _vp(self.scratch), gp))
                 lam, lam2 = lam2, lam
-        check(lib().zsc_adj_init_backward(self.ctx._h, _vp(self.phi0), _vp(lam),
-                                          _vp(self.scratch), gp))
+            del states, phis
+        check(lib().zsc_adj_init_backward(h, _vp(self.phi0), _vp(lam), _vp(self.scratch), gp))
         g = self.grads.cpu().numpy()
         g[14] = loss_slot
         self.lam0 = lam

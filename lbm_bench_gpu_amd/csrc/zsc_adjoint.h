@@ -232,7 +232,7 @@ constexpr int kAdjBlocks = 2048;
 
 // Pass 1: lam_out (local part), the three stencil-adjoint fields and
 // per-block coefficient partials.  A and lam are plain (26, n) lattices.
-template <class R>
+template <class R, bool PULL>
 __global__ void __launch_bounds__(256)
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
                 const R *__restrict__ phi, const R *__restrict__ lam, R *__restrict__ lam_out,
@@ -248,10 +248,18 @@ __global__ void __launch_bounds__(256)
         Nbr nb(x, y, z, nx, ny, nz);
         R p[19], fa[7], ga[19], lf[7], lg[19], afa[7], aga[19], fb[7], agr[3], aphi, alap;
         gather19(phi, nb, p);
+        if constexpr (PULL) {
+            // A is the post-collision lattice of the previous step: stream it
+            const Lattice<R> B = {const_cast<R *>(A), const_cast<R *>(A) + n,
+                                  const_cast<R *>(A) + 7 * n};
+            pull_f(B, n, nb, c, fa);
+            pull_g(B, n, nb, c, ga);
+        } else {
 #pragma unroll
-        for (int i = 0; i < 7; ++i) fa[i] = A[(int64_t)i * n + idx];
+            for (int i = 0; i < 7; ++i) fa[i] = A[(int64_t)i * n + idx];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) ga[i] = A[(int64_t)(7 + i) * n + idx];
+            for (int i = 0; i < 19; ++i) ga[i] = A[(int64_t)(7 + i) * n + idx];
+        }
         // S^T: gather the next adjoint from downwind
         R lfd[7];
         lf[0] = lam[idx];
@@ -403,14 +411,22 @@ struct Adj {
     {
         return align256((size_t)n * 5 * sizeof(R)) + (size_t)kAdjBlocks * kNumGrad * sizeof(double);
     }
+    // A: the step's input state, or (pulled) the previous post-collision
+    // lattice, streamed on the fly.
     static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
-                               const R *A, const R *phi, const R *lam, R *lam_out, void *scratch,
-                               double *grads)
+                               const R *A, bool pulled, const R *phi, const R *lam, R *lam_out,
+                               void *scratch, double *grads)
     {
         R *fields = (R *)scratch;
         double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
-        hipLaunchKernelGGL(k_adj_pass1<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
-                           make_coef<R>(p, grav_on), grav_on, A, phi, lam, lam_out, fields, partial);
+        if (pulled)
+            hipLaunchKernelGGL((k_adj_pass1<R, true>), dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny,
+                               g.nz, g.n, make_coef<R>(p, grav_on), grav_on, A, phi, lam, lam_out,
+                               fields, partial);
+        else
+            hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(kAdjBlocks), dim3(256), 0, s, g.nx,
+                               g.ny, g.nz, g.n, make_coef<R>(p, grav_on), grav_on, A, phi, lam,
+                               lam_out, fields, partial);
         hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            (const R *)fields, lam_out);
         hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);

@@ -681,19 +681,53 @@ extern "C" int zsc_adj_seed(zsc_ctx *c, const void *a_n, const void *phi_tgt, vo
     });
 }
 
-extern "C" int zsc_adj_backward(zsc_ctx *c, const void *a_n, const void *lam_next, void *lam_out,
-                                void *scratch, double *grads)
+extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const void *phi_n,
+                                const void *lam_next, void *lam_out, void *scratch, double *grads)
 {
-    if (!a_n || !lam_next || !lam_out || !scratch || !grads || lam_next == lam_out)
+    if (!state || !lam_next || !lam_out || !scratch || !grads || lam_next == lam_out ||
+        (pulled && !phi_n))
         return fail(ZSC_ERR_ARG, "zsc_adj_backward: null or aliased pointer");
     return adj_call(c, "zsc_adj_backward", [&](auto tag) {
         using T = ZSC_T;
         void *phi = scratch;
         void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
-        hipError_t e = Ops<T>::lat_phi(c->geo, c->stream, a_n, phi);
+        if (!phi_n) {
+            hipError_t e = Ops<T>::lat_phi(c->geo, c->stream, state, phi);
+            if (e != hipSuccess) return e;
+            phi_n = phi;
+        }
+        return Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)state,
+                                pulled != 0, (const T *)phi_n, (const T *)lam_next, (T *)lam_out,
+                                adj_scr, grads);
+    });
+}
+
+extern "C" int zsc_adj_collide(zsc_ctx *c, const void *a, void *b, void *phi)
+{
+    if (!a || !b || !phi || a == b) return fail(ZSC_ERR_ARG, "zsc_adj_collide: null or aliased pointer");
+    return adj_call(c, "zsc_adj_collide", [&](auto tag) {
+        using T = ZSC_T;
+        hipError_t e = Ops<T>::lat_phi(c->geo, c->stream, a, phi);
         if (e != hipSuccess) return e;
-        return Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)a_n,
-                                (const T *)phi, (const T *)lam_next, (T *)lam_out, adj_scr, grads);
+        return Ops<T>::lat_collide(c->geo, c->params, c->grav_on, c->stream, a, phi, b);
+    });
+}
+
+extern "C" int zsc_adj_fused(zsc_ctx *c, const void *b_in, void *b_out, void *phi)
+{
+    if (!b_in || !b_out || b_in == b_out) return fail(ZSC_ERR_ARG, "zsc_adj_fused: null or aliased pointer");
+    return adj_call(c, "zsc_adj_fused", [&](auto tag) {
+        using T = ZSC_T;
+        return Ops<T>::lat_fused(c->geo, c->params, c->grav_on, c->stream, c->cfg, b_in, b_out, phi);
+    });
+}
+
+extern "C" int zsc_adj_stream(zsc_ctx *c, const void *b, void *a)
+{
+    if (!a || !b || a == b) return fail(ZSC_ERR_ARG, "zsc_adj_stream: null or aliased pointer");
+    return adj_call(c, "zsc_adj_stream", [&](auto tag) {
+        using T = ZSC_T;
+        return Ops<T>::lat_stream(c->geo, c->params, c->stream, b, a);
     });
 }
 

@@ -479,7 +479,8 @@ __global__ void __launch_bounds__(256)
 template <class S, int TY, int FL, int MINW>
 __global__ void __launch_bounds__(64 * TY, MINW)
     k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
-            Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out)
+            Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out,
+            typename Traits<S>::R *__restrict__ phi_out)
 {
     using R = typename Traits<S>::R;
     __shared__ R sp[4][TY + 2][66];
@@ -577,6 +578,7 @@ __global__ void __launch_bounds__(64 * TY, MINW)
             R fb[7], gb[19];
             collide_cell(c, p, fcur, ga, fb, gb);
             store32<S, R, FL>(out, n, nb.at(0, 0, 0), c, fb, gb);
+            if (phi_out) phi_out[nb.at(0, 0, 0)] = p[0];  // phi of the post-stream state
         }
 #pragma unroll
         for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
@@ -729,10 +731,13 @@ hipError_t Ops<S>::lat_stream(const Geo &g, const zsc_params &p, hipStream_t s, 
 
 template <class S, int TY, int FL, int MINW>
 static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R> &c,
-                                 hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout)
+                                 hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout,
+                                 void *phi_out)
 {
-    const int zchunk = cfg.zchunk > 0 ? cfg.zchunk : g.nz;
+    int zchunk = cfg.zchunk > 0 ? cfg.zchunk : g.nz;
     const int nbx = (g.nx + 63) / 64, nby = (g.ny + TY - 1) / TY;
+    // small boxes: shorter z chunks until there are enough blocks for 256 CUs
+    while ((int64_t)nbx * nby * ((g.nz + zchunk - 1) / zchunk) < 2048 && zchunk > 4) zchunk /= 2;
     const int nbz = (g.nz + zchunk - 1) / zchunk;
     const int64_t nblk = (int64_t)nbx * nby * nbz;
     if (nblk > 0x7fffffff) return hipErrorInvalidValue;
@@ -747,26 +752,27 @@ static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R>
                        g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk,
                        (cfg.flags & kFusedIdentityMap) ? 0 : 1, c,
                        LatLayout<S>::view(const_cast<void *>(Bin), g.n),
-                       LatLayout<S>::view(Bout, g.n));
+                       LatLayout<S>::view(Bout, g.n), (typename Traits<S>::R *)phi_out);
     return hipGetLastError();
 }
 
 template <class S, int TY>
 static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R> &c,
-                                  hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout)
+                                  hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout,
+                                  void *phi)
 {
     if constexpr (sizeof(S) == 8) {
-        return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout);
+        return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout, phi);
     } else {
         const int fl = cfg.flags & (kNtStore | kNtLoad);
         const bool occ = (cfg.flags & kFusedOcc4) != 0;
         switch (fl + (occ ? 4 : 0)) {
-        case 0: return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout);
-        case 1: return launch_fused_v<S, TY, 1, 1>(g, c, s, cfg, Bin, Bout);
-        case 3: return launch_fused_v<S, TY, 3, 1>(g, c, s, cfg, Bin, Bout);
-        case 4: return launch_fused_v<S, TY, 0, 4>(g, c, s, cfg, Bin, Bout);
-        case 5: return launch_fused_v<S, TY, 1, 4>(g, c, s, cfg, Bin, Bout);
-        case 7: return launch_fused_v<S, TY, 3, 4>(g, c, s, cfg, Bin, Bout);
+        case 0: return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout, phi);
+        case 1: return launch_fused_v<S, TY, 1, 1>(g, c, s, cfg, Bin, Bout, phi);
+        case 3: return launch_fused_v<S, TY, 3, 1>(g, c, s, cfg, Bin, Bout, phi);
+        case 4: return launch_fused_v<S, TY, 0, 4>(g, c, s, cfg, Bin, Bout, phi);
+        case 5: return launch_fused_v<S, TY, 1, 4>(g, c, s, cfg, Bin, Bout, phi);
+        case 7: return launch_fused_v<S, TY, 3, 4>(g, c, s, cfg, Bin, Bout, phi);
         default: return hipErrorInvalidValue;
         }
     }
@@ -774,14 +780,14 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
 
 template <class S>
 hipError_t Ops<S>::lat_fused(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
-                             const FusedCfg &cfg, const void *Bin, void *Bout)
+                             const FusedCfg &cfg, const void *Bin, void *Bout, void *phi)
 {
     const Coef<R> c = make_coef<R>(p, grav_on);
     switch (cfg.ty) {
-    case 1: return launch_fused_ty<S, 1>(g, c, s, cfg, Bin, Bout);
-    case 2: return launch_fused_ty<S, 2>(g, c, s, cfg, Bin, Bout);
-    case 4: return launch_fused_ty<S, 4>(g, c, s, cfg, Bin, Bout);
-    case 8: return launch_fused_ty<S, 8>(g, c, s, cfg, Bin, Bout);
+    case 1: return launch_fused_ty<S, 1>(g, c, s, cfg, Bin, Bout, phi);
+    case 2: return launch_fused_ty<S, 2>(g, c, s, cfg, Bin, Bout, phi);
+    case 4: return launch_fused_ty<S, 4>(g, c, s, cfg, Bin, Bout, phi);
+    case 8: return launch_fused_ty<S, 8>(g, c, s, cfg, Bin, Bout, phi);
     default: return hipErrorInvalidValue;
     }
 }

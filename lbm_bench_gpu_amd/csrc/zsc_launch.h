@@ -63,9 +63,10 @@ struct Ops {
     // A <- stream(B).
     static hipError_t lat_stream(const Geo &, const zsc_params &, hipStream_t, const void *B,
                                  void *A);
-    // Bout <- collide(stream(Bin)): one fused time step.
+    // Bout <- collide(stream(Bin)): one fused time step; phi (R[n], may be
+    // nullptr) receives phi of the intermediate post-stream state.
     static hipError_t lat_fused(const Geo &, const zsc_params &, int grav_on, hipStream_t,
-                                const FusedCfg &, const void *Bin, void *Bout);
+                                const FusedCfg &, const void *Bin, void *Bout, void *phi = nullptr);
     // moments of A into R arrays (any may be nullptr).
     static hipError_t lat_moments(const Geo &, const zsc_params &, hipStream_t, const void *A,
                                   void *phi, void *rho, void *ux, void *uy, void *uz);

@@ -49,7 +49,8 @@ def test_step_dot_product(need_gpu):
     check(lib().zsc_adj_forward(run.ctx._h, _vp(ap), _vp(op), _vp(run.scratch)))
     check(lib().zsc_adj_forward(run.ctx._h, _vp(am), _vp(om), _vp(run.scratch)))
     gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
-    check(lib().zsc_adj_backward(run.ctx._h, _vp(a), _vp(w), _vp(lam), _vp(run.scratch), gp))
+    check(lib().zsc_adj_backward(run.ctx._h, _vp(a), 0, None, _vp(w), _vp(lam), _vp(run.scratch),
+                                 gp))
     run.ctx.sync()
     jv = (op - om) / (2 * eps)
     lhs = float((jv * w).sum())
@@ -88,21 +89,18 @@ def test_fp64_gradients_match_taped_cpu(need_gpu):
         assert abs(a - r) <= 1e-8 * max(abs(r), 1e-30), (a, r)
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_adjoint_workload_vs_central_differences(need_gpu, precision):
-    """128^3, 8 bubbles R = 12 (seed 4), 400 steps, checkpoints every 20;
-    target from kappa x1.2, beta x0.9, gravity x1.5; FD h = 1e-4 relative on
-    FP64 runs."""
+def _workload_fd(precision, gravity_on):
     import torch
     w = configs.workload(4)
     p = w.params()
-    nx, ny, nz = w.dims
     phi0 = torch.from_numpy(zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width))
     theta = adjoint.theta_of(p)
     tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
-    fd_run = adjoint.AdjointRunner(w.dims, p, precision="f64", checkpoint_every=20)
+    fd_run = adjoint.AdjointRunner(w.dims, p, precision="f64", checkpoint_every=20,
+                                   gravity_on=gravity_on)
     tgt = fd_run.forward(tt, phi0, w.steps, keep=False)[:7].sum(0).clone()
-    run = adjoint.AdjointRunner(w.dims, p, precision=precision, checkpoint_every=20)
+    run = adjoint.AdjointRunner(w.dims, p, precision=precision, checkpoint_every=20,
+                                gravity_on=gravity_on)
     L, grads = run.loss_and_grad(theta, phi0, tgt, w.steps)
     fd = []
     for i in range(5):
@@ -113,7 +111,29 @@ def test_adjoint_workload_vs_central_differences(need_gpu, precision):
         tm[i] -= h
         fd.append((fd_run.loss_only(tp, phi0, tgt, w.steps) - fd_run.loss_only(tm, phi0, tgt, w.steps))
                   / (2 * h))
-    print(precision, "loss", L, "\nadjoint", grads, "\nFD     ", fd, flush=True)
-    tol = 1e-4 if precision == "f64" else 2e-3
+    print(precision, "gravity", gravity_on, "loss", L, "\nadjoint", grads, "\nFD     ", fd, flush=True)
+    return grads, fd
+
+
+def test_adjoint_workload_smooth_vs_central_differences(need_gpu):
+    """The 128^3 workload (8 bubbles R = 12, seed 4, 400 steps, checkpoints
+    every 20, target from kappa x1.2, beta x0.9) with gravity switched off, so
+    the loss is smooth in every parameter: FP64 adjoint vs central
+    differences (h = 1e-4 relative) within 1e-6; the gravity derivative is 0."""
+    grads, fd = _workload_fd("f64", False)
+    for name, a, r in list(zip(adjoint.THETA, grads, fd))[:4]:
+        assert abs(a - r) <= 1e-6 * abs(r), (name, a, r)
+    assert grads[4] == 0.0
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 5e-4), ("f32", 3e-3)])
+def test_adjoint_workload_vs_central_differences(need_gpu, precision, tol):
+    """The 128^3 workload as specified (gravity x1.5 in the target).  The
+    phi > 0 gravity switch makes the loss piecewise smooth: central
+    differences straddle switch flips and scatter by ~3e-4 relative between
+    h = 1e-3 and 1e-5, so the bound here is that scatter (FP64) and FP32
+    accumulation error on top (FP32).  The exact comparison is against the
+    taped FP64 gradient above."""
+    grads, fd = _workload_fd(precision, True)
     for name, a, r in zip(adjoint.THETA, grads, fd):
         assert abs(a - r) <= tol * abs(r), (name, a, r)
