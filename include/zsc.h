@@ -218,8 +218,9 @@ int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
 
 /* Launch shape of the fused kernel: ty wavefront rows of 64 cells per block
  * (1, 2, 4 or 8), zchunk planes marched per block (0: the whole box) and
- * flags: 1 non-temporal output stores, 2 (with 1) non-temporal loads, 4 cap
- * registers for 4 waves per SIMD, 8 no XCD-contiguous tile order, and bits
+ * flags: 1 non-temporal output stores, 2 (with 1) non-temporal loads, 4
+ * prefetch the next plane's g populations (software pipelining), 8 no
+ * XCD-contiguous tile order, and bits
  * 8..15 a number of KiB (<= 128) of unused LDS per block that limits how
  * many blocks share a CU.  Bits 0-2 only apply to the FP32-compute
  * precisions.  No reference counterpart; a
@@ -259,6 +260,11 @@ int zsc_adj_stream(zsc_ctx *ctx, const void *b, void *a);
  * required). */
 int zsc_adj_backward(zsc_ctx *ctx, const void *state, int pulled, const void *phi_n,
                      const void *lam_next, void *lam_out, void *scratch, double *grads);
+/* With grads == NULL, zsc_adj_backward keeps the step's coefficient sums in
+ * the scratch buffer; zsc_adj_flush adds everything kept since the last
+ * flush (or zsc_adj_seed) to grads and clears it.  Saves one reduction
+ * launch per step. */
+int zsc_adj_flush(zsc_ctx *ctx, void *scratch, double *grads);
 /* grads += derivatives of the initial equilibria (alpha4, kappa) given
  * lam0 = dL/da0. */
 int zsc_adj_init_backward(zsc_ctx *ctx, const void *phi0, const void *lam0, void *scratch,

@@ -196,9 +196,10 @@ class AdjointRunner:
             for n in range(s1 - 1, s0 - 1, -1):
                 j = n - s0
                 check(lib().zsc_adj_backward(h, _vp(states[j]), 0 if n == 0 else 1, _vp(phis[j]),
-                                             _vp(lam), _vp(lam2), _vp(self.scratch), gp))
+                                             _vp(lam), _vp(lam2), _vp(self.scratch), None))
                 lam, lam2 = lam2, lam
             del states, phis
+        check(lib().zsc_adj_flush(h, _vp(self.scratch), gp))
         check(lib().zsc_adj_init_backward(h, _vp(self.phi0), _vp(lam), _vp(self.scratch), gp))
         g = self.grads.cpu().numpy()
         g[14] = loss_slot

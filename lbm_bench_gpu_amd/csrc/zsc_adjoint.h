@@ -207,7 +207,7 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
 
 // Wavefront + block reduction of kNumGrad doubles into partial[block].
 template <int NT>
-__device__ inline void block_reduce_grads(double *ap, double *partial)
+__device__ inline void block_reduce_grads(double *ap, double *partial, bool accumulate = false)
 {
     __shared__ double red[NT / 64][kNumGrad];
 #pragma unroll
@@ -224,7 +224,8 @@ __device__ inline void block_reduce_grads(double *ap, double *partial)
     if (threadIdx.x < kNumGrad) {
         double v = 0.0;
         for (int w = 0; w < NT / 64; ++w) v += red[w][threadIdx.x];
-        partial[(int64_t)blockIdx.x * kNumGrad + threadIdx.x] = v;
+        double *dst = partial + (int64_t)blockIdx.x * kNumGrad + threadIdx.x;
+        *dst = accumulate ? *dst + v : v;
     }
 }
 
@@ -236,7 +237,7 @@ template <class R, bool PULL>
 __global__ void __launch_bounds__(256)
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
                 const R *__restrict__ phi, const R *__restrict__ lam, R *__restrict__ lam_out,
-                R *__restrict__ fields, double *partial)
+                R *__restrict__ fields, double *partial, int accumulate)
 {
     double ap[kNumGrad];
 #pragma unroll
@@ -288,7 +289,7 @@ __global__ void __launch_bounds__(256)
         fields[3 * n + idx] = agr[1];
         fields[4 * n + idx] = agr[2];
     }
-    block_reduce_grads<256>(ap, partial);
+    block_reduce_grads<256>(ap, partial, accumulate != 0);
 }
 
 // Pass 2: a_phi = a_phi_local + lap[a_lap] - div[a_grad] (the transposed
@@ -407,35 +408,58 @@ __global__ void __launch_bounds__(256) k_adj_accumulate(int nblocks, const doubl
 template <class R>
 struct Adj {
     // scratch: 5n R (stencil fields) + kAdjBlocks*kNumGrad doubles (partials)
+    // scratch: 5n R (stencil fields) | partials | deferred partials
     static size_t scratch_bytes(int64_t n)
     {
-        return align256((size_t)n * 5 * sizeof(R)) + (size_t)kAdjBlocks * kNumGrad * sizeof(double);
+        return align256((size_t)n * 5 * sizeof(R)) + 2 * (size_t)kAdjBlocks * kNumGrad * sizeof(double);
+    }
+    static double *partials(const Geo &g, void *scratch, int which)
+    {
+        return (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R))) +
+               (size_t)which * kAdjBlocks * kNumGrad;
     }
     // A: the step's input state, or (pulled) the previous post-collision
     // lattice, streamed on the fly.
+    // grads == nullptr defers the coefficient sums into per-block partials
+    // kept in the scratch buffer until flush().
     static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
                                const R *A, bool pulled, const R *phi, const R *lam, R *lam_out,
                                void *scratch, double *grads)
     {
         R *fields = (R *)scratch;
-        double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
+        const int acc = grads ? 0 : 1;
+        double *partial = partials(g, scratch, acc);
         if (pulled)
             hipLaunchKernelGGL((k_adj_pass1<R, true>), dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny,
                                g.nz, g.n, make_coef<R>(p, grav_on), grav_on, A, phi, lam, lam_out,
-                               fields, partial);
+                               fields, partial, acc);
         else
             hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(kAdjBlocks), dim3(256), 0, s, g.nx,
                                g.ny, g.nz, g.n, make_coef<R>(p, grav_on), grav_on, A, phi, lam,
-                               lam_out, fields, partial);
+                               lam_out, fields, partial, acc);
         hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            (const R *)fields, lam_out);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
+        if (grads)
+            hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial,
+                               grads);
+        return hipGetLastError();
+    }
+    // grads += deferred partials; the partials are cleared.
+    static hipError_t flush(const Geo &g, hipStream_t s, void *scratch, double *grads)
+    {
+        double *d = partials(g, scratch, 1);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, d, grads);
+        hipLaunchKernelGGL(k_from_f64<double>, grid_for(kAdjBlocks * kNumGrad), dim3(256), 0, s,
+                           (int64_t)kAdjBlocks * kNumGrad, (const double *)nullptr, d);
         return hipGetLastError();
     }
     static hipError_t seed(const Geo &g, hipStream_t s, const R *A, const R *tgt, R *lam,
                            void *scratch, double *grads)
     {
-        double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
+        double *partial = partials(g, scratch, 0);
+        hipLaunchKernelGGL(k_from_f64<double>, grid_for(kAdjBlocks * kNumGrad), dim3(256), 0, s,
+                           (int64_t)kAdjBlocks * kNumGrad, (const double *)nullptr,
+                           partials(g, scratch, 1));
         hipLaunchKernelGGL(k_adj_seed<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.n, A, tgt, lam, partial);
         hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
         return hipGetLastError();
@@ -443,7 +467,7 @@ struct Adj {
     static hipError_t init_backward(const Geo &g, const zsc_params &p, hipStream_t s, const R *phi0,
                                     const R *lam, void *scratch, double *grads)
     {
-        double *partial = (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R)));
+        double *partial = partials(g, scratch, 0);
         hipLaunchKernelGGL(k_adj_init<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            make_coef<R>(p, 0), phi0, lam, partial);
         hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);

@@ -684,8 +684,7 @@ extern "C" int zsc_adj_seed(zsc_ctx *c, const void *a_n, const void *phi_tgt, vo
 extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const void *phi_n,
                                 const void *lam_next, void *lam_out, void *scratch, double *grads)
 {
-    if (!state || !lam_next || !lam_out || !scratch || !grads || lam_next == lam_out ||
-        (pulled && !phi_n))
+    if (!state || !lam_next || !lam_out || !scratch || lam_next == lam_out || (pulled && !phi_n))
         return fail(ZSC_ERR_ARG, "zsc_adj_backward: null or aliased pointer");
     return adj_call(c, "zsc_adj_backward", [&](auto tag) {
         using T = ZSC_T;
@@ -699,6 +698,16 @@ extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const
         return Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)state,
                                 pulled != 0, (const T *)phi_n, (const T *)lam_next, (T *)lam_out,
                                 adj_scr, grads);
+    });
+}
+
+extern "C" int zsc_adj_flush(zsc_ctx *c, void *scratch, double *grads)
+{
+    if (!scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_flush: null pointer");
+    return adj_call(c, "zsc_adj_flush", [&](auto tag) {
+        using T = ZSC_T;
+        return Adj<T>::flush(c->geo, c->stream, (char *)scratch + align256((size_t)c->geo.n * sizeof(T)),
+                             grads);
     });
 }
 

@@ -430,10 +430,12 @@ __global__ void __launch_bounds__(256) k_to_f64(int64_t n, const R *src, double 
     ZSC_GRID_STRIDE(idx, n) dst[idx] = (double)src[idx];
 }
 
+// dst = src converted; src == nullptr fills zeros (a kernel, so that the
+// clear can sit in a captured graph).
 template <class R>
 __global__ void __launch_bounds__(256) k_from_f64(int64_t n, const double *src, R *dst)
 {
-    ZSC_GRID_STRIDE(idx, n) dst[idx] = (R)src[idx];
+    ZSC_GRID_STRIDE(idx, n) dst[idx] = src ? (R)src[idx] : R(0);
 }
 
 // Initial phi of src/init.c:39-59 in FP64 (last bubble wins, z 1-based).
@@ -476,8 +478,8 @@ __global__ void __launch_bounds__(256)
 // sufficient: a wave writing slot (k+2)&3 can only run ahead of a wave that
 // still reads slots k-1..k+1 by one iteration.
 // ---------------------------------------------------------------------------
-template <class S, int TY, int FL, int MINW>
-__global__ void __launch_bounds__(64 * TY, MINW)
+template <class S, int TY, int FL, bool PF>
+__global__ void __launch_bounds__(64 * TY)
     k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
             Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out,
             typename Traits<S>::R *__restrict__ phi_out)
@@ -544,12 +546,27 @@ __global__ void __launch_bounds__(64 * TY, MINW)
     };
 
     R fcur[7], fnext[7];
+    R gnext[PF ? 19 : 1];
+    if constexpr (PF) {
+        if (own) pull_g32<S, R, FL>(in, n, Nbr32(x, y, z0, nx, ny, nz), c, gnext);
+    }
     plane_phi(z0 - 1, fnext);
     plane_phi(z0, fcur);
     for (int k = z0; k < z1; ++k) {
         R ga[19];
         Nbr32 nb(x, y, k, nx, ny, nz);
-        if (own) pull_g32<S, R, FL>(in, n, nb, c, ga);
+        if constexpr (PF) {
+            // software pipeline: plane k's g arrived during the previous
+            // iteration; issue plane k+1's now, to land during this one
+#pragma unroll
+            for (int i = 0; i < 19; ++i) ga[i] = gnext[i];
+            if (own && k + 1 < z1) {
+                const int kp = k + 1 == nz ? 0 : k + 1;
+                pull_g32<S, R, FL>(in, n, Nbr32(x, y, kp, nx, ny, nz), c, gnext);
+            }
+        } else {
+            if (own) pull_g32<S, R, FL>(in, n, nb, c, ga);
+        }
         plane_phi(k + 1, fnext);
         __syncthreads();
         if (own) {
@@ -729,7 +746,7 @@ hipError_t Ops<S>::lat_stream(const Geo &g, const zsc_params &p, hipStream_t s, 
     return hipGetLastError();
 }
 
-template <class S, int TY, int FL, int MINW>
+template <class S, int TY, int FL, bool PF>
 static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R> &c,
                                  hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout,
                                  void *phi_out)
@@ -748,7 +765,7 @@ static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R>
         return hipErrorInvalidValue;
     // optional dynamic LDS that only limits how many blocks share a CU
     const unsigned pad = (unsigned)((cfg.flags >> 8) & 0xff) * 1024u;
-    hipLaunchKernelGGL((k_fused<S, TY, FL, MINW>), dim3((unsigned)nblk), dim3(64, TY), pad, s,
+    hipLaunchKernelGGL((k_fused<S, TY, FL, PF>), dim3((unsigned)nblk), dim3(64, TY), pad, s,
                        g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk,
                        (cfg.flags & kFusedIdentityMap) ? 0 : 1, c,
                        LatLayout<S>::view(const_cast<void *>(Bin), g.n),
@@ -762,17 +779,17 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
                                   void *phi)
 {
     if constexpr (sizeof(S) == 8) {
-        return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout, phi);
+        return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
     } else {
         const int fl = cfg.flags & (kNtStore | kNtLoad);
-        const bool occ = (cfg.flags & kFusedOcc4) != 0;
-        switch (fl + (occ ? 4 : 0)) {
-        case 0: return launch_fused_v<S, TY, 0, 1>(g, c, s, cfg, Bin, Bout, phi);
-        case 1: return launch_fused_v<S, TY, 1, 1>(g, c, s, cfg, Bin, Bout, phi);
-        case 3: return launch_fused_v<S, TY, 3, 1>(g, c, s, cfg, Bin, Bout, phi);
-        case 4: return launch_fused_v<S, TY, 0, 4>(g, c, s, cfg, Bin, Bout, phi);
-        case 5: return launch_fused_v<S, TY, 1, 4>(g, c, s, cfg, Bin, Bout, phi);
-        case 7: return launch_fused_v<S, TY, 3, 4>(g, c, s, cfg, Bin, Bout, phi);
+        const bool pf = (cfg.flags & kFusedPrefetch) != 0;
+        switch (fl + (pf ? 4 : 0)) {
+        case 0: return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
+        case 1: return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);
+        case 3: return launch_fused_v<S, TY, 3, false>(g, c, s, cfg, Bin, Bout, phi);
+        case 4: return launch_fused_v<S, TY, 0, true>(g, c, s, cfg, Bin, Bout, phi);
+        case 5: return launch_fused_v<S, TY, 1, true>(g, c, s, cfg, Bin, Bout, phi);
+        case 7: return launch_fused_v<S, TY, 3, true>(g, c, s, cfg, Bin, Bout, phi);
         default: return hipErrorInvalidValue;
         }
     }

@@ -235,3 +235,24 @@ def test_bubble_init_on_device_matches_host(need_gpu):
     st = O.State((nz, ny, nx), oracle_coef(p)).init(phi_h)
     # device tanh may differ from libm by an ulp
     assert rel(f, st.f) < 1e-13 and rel(g, st.g) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16", "fp16"])
+def test_launch_shapes_are_bitwise_identical(need_gpu, prec):
+    """Tile rows, z chunks, streaming hints, prefetch and LDS padding change
+    only the schedule, never the arithmetic: every shape gives the same bits."""
+    dims = (70, 18, 12)
+    p, phi0 = small_case(dims, seed=13)
+    shapes = [(4, 16, 0), (8, 16, 1), (2, 3, 5), (1, 0, 4), (4, 5, 1 | (16 << 8)), (8, 2, 8)]
+    ref = None
+    for ty, zc, fl in shapes:
+        with zsc.Context(*dims, precision=prec) as ctx:
+            ctx.set_params(p)
+            ctx.set_fused_config(ty, zc, fl)
+            ctx.state_init(phi0)
+            ctx.step(9)
+            f, g = ctx.state_get()
+        if ref is None:
+            ref = (f, g)
+        else:
+            assert np.array_equal(f, ref[0]) and np.array_equal(g, ref[1]), (ty, zc, fl)
