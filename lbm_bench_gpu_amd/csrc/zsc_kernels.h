@@ -14,6 +14,10 @@
 #include "zsc_launch.h"
 #include "zsc_store.h"
 
+#ifndef ZSC_OPAQUE_STRIDE
+#define ZSC_OPAQUE_STRIDE 1
+#endif
+
 namespace zsc {
 
 template <class R>
@@ -239,6 +243,30 @@ struct Nbr32 {
     {
         return cols[1 + dx] + rows[1 + dy] + planes[1 + dz];
     }
+    __device__ uint32_t off(int dx, int dy, int dz) const { return at(dx, dy, dz); }
+    __device__ static constexpr int xstep(int) { return 0; }
+};
+
+// Neighbour offsets for a cell whose x-neighbours do not wrap.  off() gives
+// the element offset of the (dy, dz) row, x step excluded; the x step is
+// added to the pointer afterwards, where it folds into the instruction's
+// immediate offset.  The nine row offsets are wrapped once per cell.
+struct NbrRow {
+    uint32_t o[3][3];
+    __device__ NbrRow(int x, int y, int z, int nx, int ny, int nz)
+    {
+        const int32_t nxy = nx * ny;
+        const uint32_t own = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+        const int32_t ro[3] = {y == 0 ? (ny - 1) * nx : -nx, 0, y == ny - 1 ? -(ny - 1) * nx : nx};
+        const int32_t po[3] = {z == 0 ? (nz - 1) * nxy : -nxy, 0, z == nz - 1 ? -(nz - 1) * nxy : nxy};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) o[a][b] = own + (uint32_t)(ro[a] + po[b]);
+    }
+    __device__ uint32_t off(int, int dy, int dz) const { return o[1 + dy][1 + dz]; }
+    __device__ static constexpr int xstep(int dx) { return dx; }
+    __device__ uint32_t at(int dx, int dy, int dz) const { return off(dx, dy, dz) + (uint32_t)dx; }
 };
 
 template <int FL, class S, class R>
@@ -270,8 +298,16 @@ __device__ inline void st32z(S *base, uint32_t i, R v)
     st_raw<FL>(p, Codec<S>::enc0(v));
 }
 
-template <class S, class R, int FL>
-__device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const Nbr32 &nb, const Coef<R> &c,
+// element (off + xs) of base, with xs added after the 32-bit offset
+template <int FL, class S>
+__device__ inline S ldx(const S *base, uint32_t off, int xs)
+{
+    const S *p = (const S *)((const char *)base + (uint64_t)(off * (uint32_t)sizeof(S))) + xs;
+    return ld_raw<FL>(p);
+}
+
+template <class S, class R, int FL, class NB>
+__device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const NB &nb, const Coef<R> &c,
                                 R *f)
 {
     const uint32_t own = nb.at(0, 0, 0);
@@ -279,17 +315,23 @@ __device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const Nbr32 &nb,
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
         const auto *p = B.f + (int64_t)(i - 1) * n;
-        f[i] = c.eta * ld32z<FL>(p, nb.at(-cx(i), -cy(i), -cz(i))) + c.eta2 * ld32z<FL>(p, own);
+        using SF = typename Traits<S>::SF;
+        const R up = Codec<SF>::dec0(
+            ldx<FL>(p, nb.off(-cx(i), -cy(i), -cz(i)), NB::xstep(-cx(i))));
+        f[i] = c.eta * up + c.eta2 * ld32z<FL>(p, own);
     }
 }
 
-template <class S, class R, int FL>
-__device__ inline void pull_g32(const Lattice<S> &B, int64_t n, const Nbr32 &nb, const Coef<R> &c,
+template <class S, class R, int FL, class NB>
+__device__ inline void pull_g32(const Lattice<S> &B, int64_t n, const NB &nb, const Coef<R> &c,
                                 R *g)
 {
+    using SG = typename Traits<S>::SG;
 #pragma unroll
     for (int i = 0; i < 19; ++i)
-        g[i] = ld32<FL>(B.g + (int64_t)i * n, nb.at(-cx(i), -cy(i), -cz(i)), g_rest(c, i));
+        g[i] = Codec<SG>::dec(ldx<FL>(B.g + (int64_t)i * n, nb.off(-cx(i), -cy(i), -cz(i)),
+                                      NB::xstep(-cx(i))),
+                              g_rest(c, i));
 }
 
 template <class S, class R, int FL>
@@ -502,6 +544,8 @@ __global__ void __launch_bounds__(64 * TY)
     const bool own = tx < tw && ty < th;
     const int x = x0 + min(tx, tw - 1), y = y0 + min(ty, th - 1);
     const int nh = 2 * tw + 2 * (th + 2);  // ring cells per plane
+    // block-uniform: the tile's x-neighbours need no wrap
+    const bool xint = tw == 64 && x0 > 0 && x0 + 64 < nx;
     const int tid = ty * 64 + tx;
 
     // phi of plane zz (tile + ring) into LDS slot; own pulled f into fown.
@@ -509,8 +553,10 @@ __global__ void __launch_bounds__(64 * TY)
         const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
         const int slot = (zz + 4) & 3;
         if (own) {
-            Nbr32 nb(x, y, zc, nx, ny, nz);
-            pull_f32<S, R, FL>(in, n, nb, c, fown);
+            if (xint)
+                pull_f32<S, R, FL>(in, n, NbrRow(x, y, zc, nx, ny, nz), c, fown);
+            else
+                pull_f32<S, R, FL>(in, n, Nbr32(x, y, zc, nx, ny, nz), c, fown);
             sp[slot][ty + 1][tx + 1] = sum_f(fown);
         }
         for (int h = tid; h < nh; h += 64 * TY) {
@@ -548,24 +594,46 @@ __global__ void __launch_bounds__(64 * TY)
     R fcur[7], fnext[7];
     R gnext[PF ? 19 : 1];
     if constexpr (PF) {
-        if (own) pull_g32<S, R, FL>(in, n, Nbr32(x, y, z0, nx, ny, nz), c, gnext);
+        if (own) {
+            if (xint)
+                pull_g32<S, R, FL>(in, n, NbrRow(x, y, z0, nx, ny, nz), c, gnext);
+            else
+                pull_g32<S, R, FL>(in, n, Nbr32(x, y, z0, nx, ny, nz), c, gnext);
+        }
     }
     plane_phi(z0 - 1, fnext);
     plane_phi(z0, fcur);
     for (int k = z0; k < z1; ++k) {
+#if ZSC_OPAQUE_STRIDE
+        // Keep the 52 per-population base addresses out of loop-invariant
+        // hoisting (they would exceed the scalar registers and spill); the
+        // scalar unit recomputes them each plane for free.
+        {
+            uint32_t lo = (uint32_t)n;
+            asm volatile("" : "+s"(lo));
+            n = (int64_t)lo;
+        }
+#endif
         R ga[19];
-        Nbr32 nb(x, y, k, nx, ny, nz);
+        const uint32_t idx = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)k);
         if constexpr (PF) {
             // software pipeline: plane k's g arrived during the previous
             // iteration; issue plane k+1's now, to land during this one
 #pragma unroll
             for (int i = 0; i < 19; ++i) ga[i] = gnext[i];
             if (own && k + 1 < z1) {
-                const int kp = k + 1 == nz ? 0 : k + 1;
-                pull_g32<S, R, FL>(in, n, Nbr32(x, y, kp, nx, ny, nz), c, gnext);
+                if (xint)
+                    pull_g32<S, R, FL>(in, n, NbrRow(x, y, k + 1, nx, ny, nz), c, gnext);
+                else
+                    pull_g32<S, R, FL>(in, n, Nbr32(x, y, k + 1, nx, ny, nz), c, gnext);
             }
         } else {
-            if (own) pull_g32<S, R, FL>(in, n, nb, c, ga);
+            if (own) {
+                if (xint)
+                    pull_g32<S, R, FL>(in, n, NbrRow(x, y, k, nx, ny, nz), c, ga);
+                else
+                    pull_g32<S, R, FL>(in, n, Nbr32(x, y, k, nx, ny, nz), c, ga);
+            }
         }
         plane_phi(k + 1, fnext);
         __syncthreads();
@@ -594,8 +662,8 @@ __global__ void __launch_bounds__(64 * TY)
             p[18] = sp[spl][r - 1][q];
             R fb[7], gb[19];
             collide_cell(c, p, fcur, ga, fb, gb);
-            store32<S, R, FL>(out, n, nb.at(0, 0, 0), c, fb, gb);
-            if (phi_out) phi_out[nb.at(0, 0, 0)] = p[0];  // phi of the post-stream state
+            store32<S, R, FL>(out, n, idx, c, fb, gb);
+            if (phi_out) phi_out[idx] = p[0];  // phi of the post-stream state
         }
 #pragma unroll
         for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
