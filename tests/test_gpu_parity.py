@@ -237,10 +237,13 @@ def test_bubble_init_on_device_matches_host(need_gpu):
     assert rel(f, st.f) < 1e-13 and rel(g, st.g) < 1e-13
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16", "fp16"])
-def test_launch_shapes_are_bitwise_identical(need_gpu, prec):
+@pytest.mark.parametrize("prec,tol", [("f64", 0.0), ("f32", 1e-6), ("bf16", 1e-4), ("fp16", 1e-4)])
+def test_launch_shapes_agree(need_gpu, prec, tol):
     """Tile rows, z chunks, streaming hints, prefetch and LDS padding change
-    only the schedule, never the arithmetic: every shape gives the same bits."""
+    only the schedule.  FP64 (no FMA contraction) gives the same bits for
+    every shape; the FP32-compute kernels are contracted into FMAs per
+    template instance, so shapes may differ in the last FP32 bit, which a
+    16-bit store can turn into one 16-bit step: max |diff| <= tol * max |f|."""
     dims = (70, 18, 12)
     p, phi0 = small_case(dims, seed=13)
     shapes = [(4, 16, 0), (8, 16, 1), (2, 3, 5), (1, 0, 4), (4, 5, 1 | (16 << 8)), (8, 2, 8)]
@@ -255,4 +258,5 @@ def test_launch_shapes_are_bitwise_identical(need_gpu, prec):
         if ref is None:
             ref = (f, g)
         else:
-            assert np.array_equal(f, ref[0]) and np.array_equal(g, ref[1]), (ty, zc, fl)
+            ef, eg = rel(f, ref[0]), rel(g, ref[1])
+            assert ef <= tol and eg <= tol, (ty, zc, fl, ef, eg)
