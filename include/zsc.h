@@ -216,6 +216,25 @@ typedef struct zsc_stats {
  * is NaN or Inf. */
 int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
 
+/* ---- output and restart ------------------------------------------------ */
+/* Pressure of src/vtksave.c:75-119 (periodic 19-point stencil of phi, at the
+ * context's compute precision) for the A state, to an FP64 host array. */
+int zsc_state_pressure(zsc_ctx *ctx, double *p_host);
+/* Legacy VTK STRUCTURED_POINTS file of the A state in the layout of
+ * src/vtksave.c:29-141: scalars Phi and Pressure, then Vectors Velocity, x
+ * fastest, ORIGIN 1 1 1.  binary = 0 writes ASCII "%f" values like the
+ * reference; binary = 1 the legacy BINARY form (big-endian float32), for
+ * boxes where text is impractical.  Replaces vtkSave's host loops. */
+int zsc_vtk_save(zsc_ctx *ctx, const char *path, int binary);
+/* Restart file (no reference counterpart): the lattice exactly as held (post-
+ * stream or post-collision, storage precision), the box, precision,
+ * parameters, gravity switch and a caller step counter.  zsc_state_load into
+ * a context of the same box and precision restores all of them, so the run
+ * continues bit for bit as if it had not stopped; it returns ZSC_ERR_ARG for
+ * another box or precision and ZSC_ERR_IO for a damaged file. */
+int zsc_state_save(zsc_ctx *ctx, const char *path, int64_t step);
+int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
+
 /* Launch shape of the fused kernel: ty wavefront rows of 64 cells per block
  * (1, 2, 4 or 8), zchunk planes marched per block (0: the whole box) and
  * flags: 1 non-temporal output stores, 2 (with 1) non-temporal loads, 4

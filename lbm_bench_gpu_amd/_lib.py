@@ -87,6 +87,10 @@ SIGNATURES = {
     "zsc_state_export": (_I, [_VP, _PP7, _PP19]),
     "zsc_state_import": (_I, [_VP, _PP7, _PP19]),
     "zsc_state_stats": (_I, [_VP, ctypes.POINTER(ZscStats)]),
+    "zsc_state_pressure": (_I, [_VP, _DP]),
+    "zsc_vtk_save": (_I, [_VP, ctypes.c_char_p, _I]),
+    "zsc_state_save": (_I, [_VP, ctypes.c_char_p, ctypes.c_int64]),
+    "zsc_state_load": (_I, [_VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "zsc_bytes_per_cell": (_D, [_I]),
     "zsc_adj_scratch_bytes": (ctypes.c_size_t, [_VP]),
     "zsc_adj_init": (_I, [_VP, _VP, _VP]),

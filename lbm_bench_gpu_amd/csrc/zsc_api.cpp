@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <utility>
 
@@ -361,20 +362,26 @@ static int ensure_dbuf(zsc_ctx *c)
     return ZSC_OK;
 }
 
-// Brings the state into the A (post-stream) convention.
-static int materialize_a(zsc_ctx *c)
+// The post-stream (A) lattice of the current state, for reading.  A state
+// held post-collision (B) is streamed into the other lattice, which the next
+// step overwrites anyway; the state itself stays B, so reading moments,
+// statistics or populations never changes the trajectory (with 16-bit
+// storage, re-storing the streamed lattice would round it once more).
+static int a_view(zsc_ctx *c, const void **a)
 {
     if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "no state: call zsc_state_init first");
-    if (c->state == STATE_A) return ZSC_OK;
-    const int nxt = 1 - c->cur;
+    if (c->state == STATE_A) {
+        *a = c->lat[c->cur];
+        return ZSC_OK;
+    }
+    void *dst = c->lat[1 - c->cur];
     int r = dispatch(c->prec, [&](auto tag) {
         using S = ZSC_T;
-        hipError_t e = Ops<S>::lat_stream(c->geo, c->params, c->stream, c->lat[c->cur], c->lat[nxt]);
+        hipError_t e = Ops<S>::lat_stream(c->geo, c->params, c->stream, c->lat[c->cur], dst);
         return e == hipSuccess ? ZSC_OK : hip_fail(e, "stream");
     });
     if (r) return r;
-    c->cur = nxt;
-    c->state = STATE_A;
+    *a = dst;
     return ZSC_OK;
 }
 
@@ -488,7 +495,8 @@ extern "C" int zsc_step_timed(zsc_ctx *c, int nsteps, double *ms)
 extern "C" int zsc_state_get(zsc_ctx *c, double *f_host, double *g_host)
 {
     if (int r = need_ctx(c)) return r;
-    if (int r = materialize_a(c)) return r;
+    const void *a = nullptr;
+    if (int r = a_view(c, &a)) return r;
     if (int r = ensure_dbuf(c)) return r;
     const size_t bytes = (size_t)c->geo.n * sizeof(double);
     return dispatch(c->prec, [&](auto tag) {
@@ -497,7 +505,7 @@ extern "C" int zsc_state_get(zsc_ctx *c, double *f_host, double *g_host)
             double *dst = q < 7 ? (f_host ? f_host + (size_t)q * c->geo.n : nullptr)
                                 : (g_host ? g_host + (size_t)(q - 7) * c->geo.n : nullptr);
             if (!dst) continue;
-            ZSC_HIP(Ops<S>::pop_get(c->geo, c->params, c->stream, c->lat[c->cur], q, c->dbuf), "decode");
+            ZSC_HIP(Ops<S>::pop_get(c->geo, c->params, c->stream, a, q, c->dbuf), "decode");
             ZSC_HIP(hipMemcpyAsync(dst, c->dbuf, bytes, hipMemcpyDeviceToHost, c->stream), "download");
             ZSC_HIP(hipStreamSynchronize(c->stream), "download");
         }
@@ -528,10 +536,11 @@ extern "C" int zsc_state_set(zsc_ctx *c, const double *f_host, const double *g_h
 
 static int moments_to_work(zsc_ctx *c)
 {
-    if (int r = materialize_a(c)) return r;
+    const void *a = nullptr;
+    if (int r = a_view(c, &a)) return r;
     return dispatch(c->prec, [&](auto tag) {
         using S = ZSC_T;
-        ZSC_HIP(Ops<S>::lat_moments(c->geo, c->params, c->stream, c->lat[c->cur], c->w(0), c->w(1),
+        ZSC_HIP(Ops<S>::lat_moments(c->geo, c->params, c->stream, a, c->w(0), c->w(1),
                                     c->w(2), c->w(3), c->w(4)),
                 "moments");
         return ZSC_OK;
@@ -595,15 +604,16 @@ static int xport(zsc_ctx *c, void *const *f, void *const *g, const char *name)
 {
     if (int r = need_ctx(c)) return r;
     if (!all_set<void, 7>(f) || !all_set<void, 19>(g)) return fail(ZSC_ERR_ARG, "%s: null pointer", name);
+    const void *a = nullptr;
     if (Export) {
-        if (int r = materialize_a(c)) return r;
+        if (int r = a_view(c, &a)) return r;
     } else {
         if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
         if (int r = ensure_alloc(c)) return r;
     }
     return dropin(c, false, name, [&](auto tag) {
         using T = ZSC_T;
-        if (Export) return DropIn<T>::export_state(c->geo, c->stream, c->lat[c->cur], (T *const *)f, (T *const *)g);
+        if (Export) return DropIn<T>::export_state(c->geo, c->stream, a, (T *const *)f, (T *const *)g);
         hipError_t e = DropIn<T>::import_state(c->geo, c->stream, c->lat[c->cur], (T *const *)f, (T *const *)g);
         if (e == hipSuccess) c->state = STATE_A;
         return e;
@@ -618,6 +628,251 @@ extern "C" int zsc_state_export(zsc_ctx *c, void *const f[7], void *const g[19])
 extern "C" int zsc_state_import(zsc_ctx *c, void *const f[7], void *const g[19])
 {
     return xport<false>(c, f, g, "zsc_state_import");
+}
+
+// ---------------------------------------------------------------------------
+// pressure, VTK output and restart files
+// ---------------------------------------------------------------------------
+// Moments of the A state with the pressure over the density: w(0) phi,
+// w(1) pressure, w(2..4) u.
+static int moments_and_pressure(zsc_ctx *c)
+{
+    if (int r = moments_to_work(c)) return r;
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        ZSC_HIP(Ops<S>::pressure(c->geo, c->params, c->stream, c->w(0), c->w(1)), "pressure");
+        return ZSC_OK;
+    });
+}
+
+extern "C" int zsc_state_pressure(zsc_ctx *c, double *p_host)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!p_host) return fail(ZSC_ERR_ARG, "zsc_state_pressure: null output");
+    if (int r = moments_and_pressure(c)) return r;
+    if (int r = ensure_dbuf(c)) return r;
+    return dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        ZSC_HIP(Ops<S>::to_f64(c->stream, c->geo.n, c->w(1), c->dbuf), "convert");
+        ZSC_HIP(hipMemcpyAsync(p_host, c->dbuf, (size_t)c->geo.n * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream),
+                "download");
+        ZSC_HIP(hipStreamSynchronize(c->stream), "download");
+        return ZSC_OK;
+    });
+}
+
+namespace {
+
+// Big-endian float32 for legacy VTK BINARY sections.
+inline uint32_t be_f32(double v)
+{
+    const float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return __builtin_bswap32(u);
+}
+
+template <class R>
+double as_double(const char *base, int64_t i)
+{
+    R v;
+    memcpy(&v, base + i * (int64_t)sizeof(R), sizeof(R));
+    return (double)v;
+}
+
+struct FileCloser {
+    FILE *fp;
+    ~FileCloser()
+    {
+        if (fp) fclose(fp);
+    }
+};
+
+}  // namespace
+
+// Writes fields of the work arrays in slabs of whole z planes: ncomp = 1 for
+// one scalar (array i0), 3 for the interleaved vector (arrays i0..i0+2).
+template <class R>
+static int vtk_field(zsc_ctx *c, FILE *fp, int i0, int ncomp, int binary)
+{
+    const int64_t n = c->geo.n, plane = (int64_t)c->geo.nx * c->geo.ny;
+    int64_t slab = ((int64_t)1 << 22) / plane;
+    if (slab < 1) slab = 1;
+    const int64_t cells_max = slab * plane;
+    std::string host((size_t)(cells_max * ncomp) * sizeof(R), '\0');
+    std::string out;
+    for (int64_t c0 = 0; c0 < n; c0 += cells_max) {
+        const int64_t m = std::min(cells_max, n - c0);
+        for (int k = 0; k < ncomp; ++k)
+            ZSC_HIP(hipMemcpyAsync(&host[(size_t)(k * cells_max) * sizeof(R)],
+                                   (const char *)c->w(i0 + k) + c0 * (int64_t)sizeof(R),
+                                   (size_t)m * sizeof(R), hipMemcpyDeviceToHost, c->stream),
+                    "vtk download");
+        ZSC_HIP(hipStreamSynchronize(c->stream), "vtk download");
+        const char *hb = host.data();
+        if (binary) {
+            out.resize((size_t)(m * ncomp) * 4);
+            uint32_t *o = (uint32_t *)&out[0];
+            for (int64_t i = 0; i < m; ++i)
+                for (int k = 0; k < ncomp; ++k)
+                    o[i * ncomp + k] = be_f32(as_double<R>(hb, k * cells_max + i));
+            if (fwrite(out.data(), 1, out.size(), fp) != out.size())
+                return fail(ZSC_ERR_IO, "zsc_vtk_save: write failed");
+        } else {
+            out.clear();
+            char buf[128];
+            for (int64_t i = 0; i < m; ++i) {
+                int len;
+                if (ncomp == 1)
+                    len = snprintf(buf, sizeof buf, "%f\n", as_double<R>(hb, i));
+                else
+                    len = snprintf(buf, sizeof buf, "%f %f %f\n", as_double<R>(hb, i),
+                                   as_double<R>(hb, cells_max + i), as_double<R>(hb, 2 * cells_max + i));
+                out.append(buf, (size_t)len);
+            }
+            if (fwrite(out.data(), 1, out.size(), fp) != out.size())
+                return fail(ZSC_ERR_IO, "zsc_vtk_save: write failed");
+        }
+    }
+    return ZSC_OK;
+}
+
+extern "C" int zsc_vtk_save(zsc_ctx *c, const char *path, int binary)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!path) return fail(ZSC_ERR_ARG, "zsc_vtk_save: null path");
+    if (int r = moments_and_pressure(c)) return r;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(ZSC_ERR_IO, "zsc_vtk_save: cannot open %s", path);
+    FileCloser closer{fp};
+    const Geo &g = c->geo;
+    // header of src/vtksave.c:44-57 (one rank: the z origin is plane 1)
+    fprintf(fp, "# vtk DataFile Version 2.0\n");
+    fprintf(fp, "libzsc ZSC bubbly flow\n");
+    fprintf(fp, binary ? "BINARY\n" : "ASCII\n");
+    fprintf(fp, "\nDATASET STRUCTURED_POINTS\n");
+    fprintf(fp, "DIMENSIONS %d %d %d\n", g.nx, g.ny, g.nz);
+    fprintf(fp, "ORIGIN %d %d %d\n", 1, 1, 1);
+    fprintf(fp, "SPACING %d %d %d\n", 1, 1, 1);
+    fprintf(fp, "\nPOINT_DATA %lld\n", (long long)g.n);
+    auto body = [&](auto rtag) {
+        using R = typename decltype(rtag)::type;
+        fprintf(fp, "\nSCALARS Phi float\nLOOKUP_TABLE default\n");
+        if (int r = vtk_field<R>(c, fp, 0, 1, binary)) return r;
+        fprintf(fp, "\nSCALARS Pressure float\nLOOKUP_TABLE default\n");
+        if (int r = vtk_field<R>(c, fp, 1, 1, binary)) return r;
+        fprintf(fp, "\nVectors Velocity float\n");
+        return vtk_field<R>(c, fp, 2, 3, binary);
+    };
+    int r = c->prec == ZSC_F64 ? body(Tag<double>{}) : body(Tag<float>{});
+    if (r) return r;
+    if (binary) fputc('\n', fp);
+    if (ferror(fp)) return fail(ZSC_ERR_IO, "zsc_vtk_save: write failed");
+    return ZSC_OK;
+}
+
+namespace {
+// Restart file header, host byte order, followed by the lattice bytes.
+struct RestartHeader {
+    char magic[8];
+    int32_t version, precision;
+    int32_t nx, ny, nz;
+    int32_t state, grav_on, pad;
+    int64_t step;
+    uint64_t lattice_bytes;
+    zsc_params params;
+};
+const char kRestartMagic[8] = {'Z', 'S', 'C', 'S', 'T', 'A', 'T', 'E'};
+const int64_t kRestartChunk = (int64_t)64 << 20;
+}  // namespace
+
+extern "C" int zsc_state_save(zsc_ctx *c, const char *path, int64_t step)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!path) return fail(ZSC_ERR_ARG, "zsc_state_save: null path");
+    if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "zsc_state_save: no state");
+    RestartHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, kRestartMagic, 8);
+    h.version = 1;
+    h.precision = c->prec;
+    h.nx = c->geo.nx;
+    h.ny = c->geo.ny;
+    h.nz = c->geo.nz;
+    h.state = c->state;
+    h.grav_on = c->grav_on;
+    h.step = step;
+    h.lattice_bytes = c->lat_bytes;
+    h.params = c->params;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(ZSC_ERR_IO, "zsc_state_save: cannot open %s", path);
+    FileCloser closer{fp};
+    if (fwrite(&h, sizeof h, 1, fp) != 1) return fail(ZSC_ERR_IO, "zsc_state_save: write failed");
+    void *pin = nullptr;
+    ZSC_HIP(hipHostMalloc(&pin, (size_t)kRestartChunk, hipHostMallocDefault), "pinned buffer");
+    int r = ZSC_OK;
+    const char *src = (const char *)c->lat[c->cur];
+    for (int64_t off = 0; off < (int64_t)c->lat_bytes && r == ZSC_OK; off += kRestartChunk) {
+        const size_t m = (size_t)std::min(kRestartChunk, (int64_t)c->lat_bytes - off);
+        hipError_t e = hipMemcpyAsync(pin, src + off, m, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+            r = hip_fail(e, "zsc_state_save: download");
+        else if (fwrite(pin, 1, m, fp) != m)
+            r = fail(ZSC_ERR_IO, "zsc_state_save: write failed");
+    }
+    (void)hipHostFree(pin);
+    return r;
+}
+
+extern "C" int zsc_state_load(zsc_ctx *c, const char *path, int64_t *step)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!path) return fail(ZSC_ERR_ARG, "zsc_state_load: null path");
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail(ZSC_ERR_IO, "zsc_state_load: cannot open %s", path);
+    FileCloser closer{fp};
+    RestartHeader h;
+    if (fread(&h, sizeof h, 1, fp) != 1 || memcmp(h.magic, kRestartMagic, 8) != 0 || h.version != 1)
+        return fail(ZSC_ERR_IO, "zsc_state_load: %s is not a libzsc restart file", path);
+    if (h.precision != c->prec || h.nx != c->geo.nx || h.ny != c->geo.ny || h.nz != c->geo.nz)
+        return fail(ZSC_ERR_ARG,
+                    "zsc_state_load: file holds a %dx%dx%d box at precision %d, context is "
+                    "%dx%dx%d at %d",
+                    h.nx, h.ny, h.nz, h.precision, c->geo.nx, c->geo.ny, c->geo.nz, c->prec);
+    if (h.state != STATE_A && h.state != STATE_B)
+        return fail(ZSC_ERR_IO, "zsc_state_load: bad state kind %d", h.state);
+    if (int r = ensure_alloc(c)) return r;
+    if (h.lattice_bytes != c->lat_bytes)
+        return fail(ZSC_ERR_IO, "zsc_state_load: lattice size %llu, expected %llu",
+                    (unsigned long long)h.lattice_bytes, (unsigned long long)c->lat_bytes);
+    void *pin = nullptr;
+    ZSC_HIP(hipHostMalloc(&pin, (size_t)kRestartChunk, hipHostMallocDefault), "pinned buffer");
+    int r = ZSC_OK;
+    char *dst = (char *)c->lat[0];
+    for (int64_t off = 0; off < (int64_t)c->lat_bytes && r == ZSC_OK; off += kRestartChunk) {
+        const size_t m = (size_t)std::min(kRestartChunk, (int64_t)c->lat_bytes - off);
+        if (fread(pin, 1, m, fp) != m) {
+            r = fail(ZSC_ERR_IO, "zsc_state_load: %s is truncated", path);
+            break;
+        }
+        hipError_t e = hipMemcpyAsync(dst + off, pin, m, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = hip_fail(e, "zsc_state_load: upload");
+    }
+    (void)hipHostFree(pin);
+    if (r) {
+        c->state = STATE_NONE;
+        return r;
+    }
+    c->cur = 0;
+    c->state = h.state;
+    c->params = h.params;
+    c->have_params = true;
+    c->grav_on = h.grav_on;
+    if (step) *step = h.step;
+    return ZSC_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -57,6 +57,30 @@ template <> struct W<float> {
                            w2 = 1.0f / 36.0f;
 };
 
+// Quotients of the step.  FP64 divides exactly, which FP64 parity needs;
+// FP32 multiplies by the hardware reciprocal (v_rcp_f32, 1 ulp), as the
+// reference's FP32 build approximates its divisions (-use_fast_math,
+// src/Makefile:5), and folds the constant 1/6.
+template <class R>
+struct Q {
+    __host__ __device__ static R recip(R b) { return R(1) / b; }
+    __host__ __device__ static R div(R a, R b) { return a / b; }
+    __host__ __device__ static R div6(R a) { return a / R(6); }
+};
+template <>
+struct Q<float> {
+    __host__ __device__ static float recip(float b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_rcpf(b);
+#else
+        return 1.0f / b;
+#endif
+    }
+    __host__ __device__ static float div(float a, float b) { return a * recip(b); }
+    __host__ __device__ static float div6(float a) { return a * (1.0f / 6.0f); }
+};
+
 // Rest value of g_i used by the 16-bit deviation codecs: w_i * rho_mean.
 template <class R>
 __host__ __device__ inline R g_weight(int i)
@@ -97,6 +121,25 @@ __host__ __device__ inline void force19(const Coef<R> &c, const R *p, R &mu, R &
     if (phin > R(0)) fz = fz + c.grav;
 }
 
+// Pressure of src/vtksave.c:75-119 (the same expression as src/logsave.c:
+// 84-137): alpha (phi^2 (3 phi^2 - 2 phi*^2) - phi*^4)
+// - kappa (phi lap(phi) - |grad phi|^2 / 2) + cs^2 rho.
+template <class R>
+__host__ __device__ inline R pressure19(R alpha, R phi_star_sq, R kappa, const R *p, R rho)
+{
+    const R phin = p[0], phin2 = phin * phin;
+    const R lap = lap19(p);
+    const R gx = (R(2) * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] + p[13] -
+                  p[14]) * W<R>::inv12;
+    const R gy = (R(2) * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] + p[17] -
+                  p[18]) * W<R>::inv12;
+    const R gz = (R(2) * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] + p[18] -
+                  p[17]) * W<R>::inv12;
+    const R gsq = gx * gx + gy * gy + gz * gz;
+    return alpha * (phin2 * (R(3) * phin2 - R(2) * phi_star_sq) - phi_star_sq * phi_star_sq) -
+           kappa * (phin * lap - R(0.5) * gsq) + W<R>::cs_sq * rho;
+}
+
 // phi = f0 + ... + f6 (src/update_phi.cu:42-43).
 template <class R>
 __host__ __device__ inline R sum_f(const R *f)
@@ -118,7 +161,7 @@ __host__ __device__ inline R sum_g(const R *g)
 template <class R>
 __host__ __device__ inline void velocity(const R *g, R rho, R &ux, R &uy, R &uz)
 {
-    const R inv_rho = R(1) / rho;
+    const R inv_rho = Q<R>::recip(rho);
     ux = (g[1] - g[2] + g[7] - g[8] + g[9] - g[10] + g[11] - g[12] + g[13] - g[14]) * inv_rho;
     uy = (g[3] - g[4] + g[7] - g[8] - g[9] + g[10] + g[15] - g[16] + g[17] - g[18]) * inv_rho;
     uz = (g[5] - g[6] + g[11] - g[12] - g[13] + g[14] + g[15] - g[16] - g[17] + g[18]) * inv_rho;
@@ -129,7 +172,7 @@ template <class R>
 __host__ __device__ inline void collide_f(const Coef<R> &c, R phin, R rho, R ux, R uy, R uz,
                                           R mu, R fx, R fy, R fz, const R *fa, R *fb)
 {
-    const R inv_rho = R(1) / rho;
+    const R inv_rho = Q<R>::recip(rho);
     const R uxn = ux + R(0.5) * fx * inv_rho;
     const R uyn = uy + R(0.5) * fy * inv_rho;
     const R uzn = uz + R(0.5) * fz * inv_rho;
@@ -153,9 +196,9 @@ __host__ __device__ inline void collide_g(const Coef<R> &c, R phin, R rhon, R ux
                                           R mu, R fx, R fy, R fz, const R *ga, R *gb)
 {
     const R K1 = c.k1, K2 = c.k2, KC1 = c.kc1, KC2 = c.kc2, t = c.inv_tau_rho_one;
-    R uxn = ux + R(0.5) * fx / rhon;
-    R uyn = uy + R(0.5) * fy / rhon;
-    R uzn = uz + R(0.5) * fz / rhon;
+    R uxn = ux + Q<R>::div(R(0.5) * fx, rhon);
+    R uyn = uy + Q<R>::div(R(0.5) * fy, rhon);
+    R uzn = uz + Q<R>::div(R(0.5) * fz, rhon);
     R temp = K1 * (R(3) * phin * mu + rhon * (R(1) - R(1.5) * (uxn * uxn + uyn * uyn + uzn * uzn)));
     const R uf = uxn * fx + uyn * fy + uzn * fz;
     uxn = uxn * W<R>::inv_cs_sq;
@@ -164,7 +207,7 @@ __host__ __device__ inline void collide_g(const Coef<R> &c, R phin, R rhon, R ux
     R eq1, eq2;
 
     gb[0] = t * ga[0] +
-            c.k0 * (rhon - R(6) * phin * mu - rhon * (uxn * uxn + uyn * uyn + uzn * uzn) / R(6)) -
+            c.k0 * (rhon - R(6) * phin * mu - Q<R>::div6(rhon * (uxn * uxn + uyn * uyn + uzn * uzn))) -
             c.kc0 * uf;
 
     eq1 = temp + K1 * rhon * uxn * uxn * R(0.5) + KC1 * (uxn * fx - uf);
@@ -182,7 +225,7 @@ __host__ __device__ inline void collide_g(const Coef<R> &c, R phin, R rhon, R ux
     gb[5] = t * ga[5] + eq1 + eq2;
     gb[6] = t * ga[6] + eq1 - eq2;
 
-    temp = K2 * (R(3) * phin * mu + rhon * (R(1) - (uxn * uxn + uyn * uyn + uzn * uzn) / R(6)));
+    temp = K2 * (R(3) * phin * mu + rhon * (R(1) - Q<R>::div6(uxn * uxn + uyn * uyn + uzn * uzn)));
 
     auto diag = [&](int ip, int im, R s, R S) {
         const R e1 = temp + K2 * rhon * s * R(0.5) * s + KC2 * (s * S - uf);
@@ -196,6 +239,69 @@ __host__ __device__ inline void collide_g(const Coef<R> &c, R phin, R rhon, R ux
     diag(13, 14, uxn - uzn, fx - fz);
     diag(15, 16, uyn + uzn, fy + fz);
     diag(17, 18, uyn - uzn, fy - fz);
+}
+
+// collide_g on deviations dg_i = g_i - w_i rho_mean (16-bit storage).
+// Because K_i = w_i / tau and 1 - t = 1 / tau, the rest state cancels
+// exactly: each "rho * 1" term of the equilibria becomes drho = rho -
+// rho_mean and g_B - w_i rho_mean = t dg_A + (the same equilibrium with drho).
+// Terms that multiply rho by u or F keep the full rho.
+template <class R>
+__host__ __device__ inline void collide_g_dev(const Coef<R> &c, R phin, R rhon, R drho, R ux, R uy,
+                                              R uz, R mu, R fx, R fy, R fz, const R *ga, R *gb)
+{
+    const R K1 = c.k1, K2 = c.k2, KC1 = c.kc1, KC2 = c.kc2, t = c.inv_tau_rho_one;
+    R uxn = ux + Q<R>::div(R(0.5) * fx, rhon);
+    R uyn = uy + Q<R>::div(R(0.5) * fy, rhon);
+    R uzn = uz + Q<R>::div(R(0.5) * fz, rhon);
+    R temp = K1 * (R(3) * phin * mu + drho - rhon * R(1.5) * (uxn * uxn + uyn * uyn + uzn * uzn));
+    const R uf = uxn * fx + uyn * fy + uzn * fz;
+    uxn = uxn * W<R>::inv_cs_sq;
+    uyn = uyn * W<R>::inv_cs_sq;
+    uzn = uzn * W<R>::inv_cs_sq;
+    R eq1, eq2;
+    gb[0] = t * ga[0] +
+            c.k0 * (drho - R(6) * phin * mu - Q<R>::div6(rhon * (uxn * uxn + uyn * uyn + uzn * uzn))) -
+            c.kc0 * uf;
+    eq1 = temp + K1 * rhon * uxn * uxn * R(0.5) + KC1 * (uxn * fx - uf);
+    eq2 = uxn * (K1 * rhon + KC1 * fx);
+    gb[1] = t * ga[1] + eq1 + eq2;
+    gb[2] = t * ga[2] + eq1 - eq2;
+    eq1 = temp + K1 * rhon * uyn * R(0.5) * uyn + KC1 * (uyn * fy - uf);
+    eq2 = uyn * (K1 * rhon + KC1 * fy);
+    gb[3] = t * ga[3] + eq1 + eq2;
+    gb[4] = t * ga[4] + eq1 - eq2;
+    eq1 = temp + K1 * rhon * uzn * R(0.5) * uzn + KC1 * (uzn * fz - uf);
+    eq2 = uzn * (K1 * rhon + KC1 * fz);
+    gb[5] = t * ga[5] + eq1 + eq2;
+    gb[6] = t * ga[6] + eq1 - eq2;
+    temp = K2 * (R(3) * phin * mu + drho - Q<R>::div6(rhon * (uxn * uxn + uyn * uyn + uzn * uzn)));
+    auto diag = [&](int ip, int im, R s, R S) {
+        const R e1 = temp + K2 * rhon * s * R(0.5) * s + KC2 * (s * S - uf);
+        const R e2 = K2 * rhon * s + KC2 * S;
+        gb[ip] = t * ga[ip] + e1 + e2;
+        gb[im] = t * ga[im] + e1 - e2;
+    };
+    diag(7, 8, uxn + uyn, fx + fy);
+    diag(9, 10, uxn - uyn, fx - fy);
+    diag(11, 12, uxn + uzn, fx + fz);
+    diag(13, 14, uxn - uzn, fx - fz);
+    diag(15, 16, uyn + uzn, fy + fz);
+    diag(17, 18, uyn - uzn, fy - fz);
+}
+
+// collide_cell with g given and returned as deviations from w_i rho_mean.
+template <class R>
+__host__ __device__ inline void collide_cell_dev(const Coef<R> &c, const R *p, const R *fa,
+                                                 const R *dga, R *fb, R *dgb)
+{
+    R mu, fx, fy, fz, ux, uy, uz;
+    const R drho = sum_g(dga);
+    const R rho = c.rho_mean + drho;
+    velocity(dga, rho, ux, uy, uz);  // sum_i c_i w_i = 0: the rest state carries no momentum
+    force19(c, p, mu, fx, fy, fz);
+    collide_f(c, p[0], rho, ux, uy, uz, mu, fx, fy, fz, fa, fb);
+    collide_g_dev(c, p[0], rho, drho, ux, uy, uz, mu, fx, fy, fz, dga, dgb);
 }
 
 // Equilibria at u = 0 of src/init_f.cu:69-78 and src/init_g.cu:74-91.

@@ -322,27 +322,49 @@ __device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const NB &nb, co
     }
 }
 
+// Pulled g of one cell; for deviation storage the values stay deviations
+// (the fused kernel collides them in that form).
 template <class S, class R, int FL, class NB>
 __device__ inline void pull_g32(const Lattice<S> &B, int64_t n, const NB &nb, const Coef<R> &c,
                                 R *g)
 {
     using SG = typename Traits<S>::SG;
 #pragma unroll
-    for (int i = 0; i < 19; ++i)
-        g[i] = Codec<SG>::dec(ldx<FL>(B.g + (int64_t)i * n, nb.off(-cx(i), -cy(i), -cz(i)),
-                                      NB::xstep(-cx(i))),
-                              g_rest(c, i));
+    for (int i = 0; i < 19; ++i) {
+        const SG v = ldx<FL>(B.g + (int64_t)i * n, nb.off(-cx(i), -cy(i), -cz(i)), NB::xstep(-cx(i)));
+        if constexpr (Codec<SG>::deviation)
+            g[i] = Codec<SG>::dec0(v);
+        else
+            g[i] = Codec<SG>::dec(v, g_rest(c, i));
+    }
+}
+
+// Population base address held in scalar registers: the stores then take
+// the scalar-base + 32-bit vector-offset form, instead of a 64-bit vector
+// address chain (one VALU op per population).
+template <class T>
+__device__ inline T *sbase(T *p, int64_t elems)
+{
+    int64_t b = elems * (int64_t)sizeof(T);
+    asm volatile("" : "+s"(b));
+    return (T *)((char *)p + b);
 }
 
 template <class S, class R, int FL>
 __device__ inline void store32(const Lattice<S> &L, int64_t n, uint32_t idx, const Coef<R> &c,
                                const R *f, const R *g)
 {
+    using SG = typename Traits<S>::SG;
     st32z<FL>(L.f0, idx, f[0]);
 #pragma unroll
-    for (int i = 1; i < 7; ++i) st32z<FL>(L.f + (int64_t)(i - 1) * n, idx, f[i]);
+    for (int i = 1; i < 7; ++i) st32z<FL>(sbase(L.f, (int64_t)(i - 1) * n), idx, f[i]);
 #pragma unroll
-    for (int i = 0; i < 19; ++i) st32<FL>(L.g + (int64_t)i * n, idx, g[i], g_rest(c, i));
+    for (int i = 0; i < 19; ++i) {
+        if constexpr (Codec<SG>::deviation)
+            st32z<FL>(sbase(L.g, (int64_t)i * n), idx, g[i]);  // g holds deviations
+        else
+            st32<FL>(sbase(L.g, (int64_t)i * n), idx, g[i], g_rest(c, i));
+    }
 }
 
 template <class S>
@@ -546,6 +568,8 @@ __global__ void __launch_bounds__(64 * TY)
     const int nh = 2 * tw + 2 * (th + 2);  // ring cells per plane
     // block-uniform: the tile's x-neighbours need no wrap
     const bool xint = tw == 64 && x0 > 0 && x0 + 64 < nx;
+    // the ring cells (x0-1 .. x0+64) pull from x0-2 .. x0+65
+    const bool xint_ring = tw == 64 && x0 >= 2 && x0 + 66 <= nx;
     const int tid = ty * 64 + tx;
 
     // phi of plane zz (tile + ring) into LDS slot; own pulled f into fown.
@@ -584,9 +608,11 @@ __global__ void __launch_bounds__(64 * TY)
             }
             hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
             hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
-            Nbr32 nb(hx, hy, zc, nx, ny, nz);
             R f[7];
-            pull_f32<S, R, FL>(in, n, nb, c, f);
+            if (xint_ring)
+                pull_f32<S, R, FL>(in, n, NbrRow(hx, hy, zc, nx, ny, nz), c, f);
+            else
+                pull_f32<S, R, FL>(in, n, Nbr32(hx, hy, zc, nx, ny, nz), c, f);
             sp[slot][r][col] = sum_f(f);
         }
     };
@@ -661,7 +687,10 @@ __global__ void __launch_bounds__(64 * TY)
             p[17] = sp[sm][r + 1][q];
             p[18] = sp[spl][r - 1][q];
             R fb[7], gb[19];
-            collide_cell(c, p, fcur, ga, fb, gb);
+            if constexpr (Codec<typename Traits<S>::SG>::deviation)
+                collide_cell_dev(c, p, fcur, ga, fb, gb);
+            else
+                collide_cell(c, p, fcur, ga, fb, gb);
             store32<S, R, FL>(out, n, idx, c, fb, gb);
             if (phi_out) phi_out[idx] = p[0];  // phi of the post-stream state
         }
@@ -694,18 +723,7 @@ __global__ void __launch_bounds__(256)
         double p[19];
 #pragma unroll
         for (int d = 0; d < 19; ++d) p[d] = (double)phi[nb.at(cx(d), cy(d), cz(d))];
-        const double phin2 = ph * ph;
-        const double lap = lap19(p);
-        const double gx = (2.0 * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] +
-                           p[13] - p[14]) * W<double>::inv12;
-        const double gy = (2.0 * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] +
-                           p[17] - p[18]) * W<double>::inv12;
-        const double gz = (2.0 * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] +
-                           p[18] - p[17]) * W<double>::inv12;
-        const double gsq = gx * gx + gy * gy + gz * gz;
-        const double pres = alpha * (phin2 * (3.0 * phin2 - 2.0 * phi_star_sq) -
-                                     phi_star_sq * phi_star_sq) -
-                            kappa * (ph * lap - 0.5 * gsq) + W<double>::cs_sq * (double)rho[idx];
+        const double pres = pressure19(alpha, phi_star_sq, kappa, p, (double)rho[idx]);
         if (ph >= 0.0) {
             a[2] += 1.0;
             a[3] += vx;
@@ -737,6 +755,23 @@ __global__ void __launch_bounds__(256)
         double v = red[0][j];
         for (int w = 1; w < 4; ++w) v = (j == 6) ? (red[w][j] > v ? red[w][j] : v) : v + red[w][j];
         partial[(int64_t)blockIdx.x * 10 + j] = v;
+    }
+}
+
+// Pressure field at precision R, written over the density array (each cell
+// reads only its own rho, so the update in place is race-free).
+template <class R>
+__global__ void __launch_bounds__(256)
+    k_pressure(int nx, int ny, int nz, int64_t n, R alpha, R phi_star_sq, R kappa,
+               const R *__restrict__ phi, R *rho_p)
+{
+    ZSC_GRID_STRIDE(idx, n)
+    {
+        int x, y, z;
+        cell_xyz(idx, nx, ny, x, y, z);
+        R p[19];
+        gather19(phi, Nbr(x, y, z, nx, ny, nz), p);
+        rho_p[idx] = pressure19(alpha, phi_star_sq, kappa, p, rho_p[idx]);
     }
 }
 
@@ -959,6 +994,15 @@ hipError_t Ops<S>::stats(const Geo &g, const zsc_params &p, hipStream_t s, const
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_stats_final<R>, dim3(1), dim3(256), 0, s, kStatsBlocks, partial, out);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t Ops<S>::pressure(const Geo &g, const zsc_params &p, hipStream_t s, const void *phi,
+                            void *rho_p)
+{
+    hipLaunchKernelGGL(k_pressure<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+                       (R)p.alpha, (R)p.phi_star_sq, (R)p.kappa, (const R *)phi, (R *)rho_p);
     return hipGetLastError();
 }
 

@@ -81,6 +81,9 @@ struct Ops {
     // phi (R[n]) from bubbles (device FP64 [nb][4]).
     static hipError_t init_phi_bubbles(const Geo &, hipStream_t, const double *bubbles, int nb,
                                        double phi_star, double width, void *phi);
+    // rho_p (R[n]) <- pressure from phi and rho_p (the density on entry).
+    static hipError_t pressure(const Geo &, const zsc_params &, hipStream_t, const void *phi,
+                               void *rho_p);
     // FP64 reductions of the moment arrays; out (device) gets 10 doubles in
     // zsc_stats order, partial needs stats_partials() * 10 doubles.
     static hipError_t stats(const Geo &, const zsc_params &, hipStream_t, const void *phi,

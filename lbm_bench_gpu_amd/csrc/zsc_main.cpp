@@ -1,21 +1,27 @@
 // zsc_main.cpp -- zsc_run, the host driver of libzsc: the time loop of the
 // reference's main() (src/main.cu:83-667) over the C-ABI, on one GPU.
 //
-//   zsc_run [-d DIR] [-p f64|f32|bf16|fp16]
+//   zsc_run [-d DIR] [-p f64|f32|bf16|fp16] [-v ascii|binary|none]
+//           [-r RESTART_IN] [-s RESTART_OUT]
 //
 // Reads DIR/properties.in and DIR/discrete.in (default: the current
 // directory), runs relaxStep steps with gravity off and maxStep-1 steps with
 // gravity on (src/main.cu:363,496), writes relax_stats.out / stats.out every
-// `stat` steps in the format of src/stats.c:73,91 and a runlog.out with the
-// sections of src/logsave.c:181-281.  Differences from the reference: the
-// statistics are reduced on the GPU in FP64 from the current post-stream
-// state (the reference uses moments one step old); VTK output is not written.
+// `stat` steps in the format of src/stats.c:73,91, VTK files every `save`
+// steps named as src/vtksave.c:39-40 (relax_data_00_NNNNNNN.vtk,
+// data_00_NNNNNNN.vtk), and a runlog.out with the sections of
+// src/logsave.c:181-281, including the Laplace-law check after relaxation
+// (src/logsave.c:155-162).  -r resumes the evolution phase from a restart
+// file (no relaxation); -s writes one at the end.  Differences from the
+// reference: the statistics are reduced on the GPU in FP64 from the current
+// post-stream state (the reference uses moments one step old).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -52,13 +58,61 @@ static void stats_line(const StatsFile &sf, int step, const zsc_stats &s, double
     fclose(fp);
 }
 
+// VTK file of src/vtksave.c:39-40 for one rank (vproc = 0).
+static int vtk(zsc_ctx *ctx, const std::string &dir, int mode, bool relax, int step)
+{
+    if (mode < 0) return ZSC_OK;
+    char name[64];
+    snprintf(name, sizeof name, relax ? "relax_data_%02d_%07d.vtk" : "data_%02d_%07d.vtk", 0, step);
+    return zsc_vtk_save(ctx, (dir + "/" + name).c_str(), mode);
+}
+
+// Runs steps first..last (inclusive) in chunks that end on the stat and save
+// intervals, with stats lines and VTK files there.
+static int run_phase(zsc_ctx *ctx, const zsc_inputs &in, const std::string &dir, int vmode,
+                     bool relax, const StatsFile &sf, double inv_vol, int first, int last,
+                     double *kernel_ms)
+{
+    for (int step = first; step <= last;) {
+        // the first multiple of stat or save at or after this step
+        int next = std::min(last, (step + in.stat - 1) / in.stat * in.stat);
+        next = std::min(next, (step + in.save - 1) / in.save * in.save);
+        const int chunk = next - step + 1;
+        double ms = 0.0;
+        if (int r = zsc_step_timed(ctx, chunk, &ms)) return die("zsc_step", r);
+        if (kernel_ms) *kernel_ms += ms;
+        step += chunk;
+        const int done = step - 1;
+        if (done % in.stat == 0) {
+            zsc_stats s;
+            if (int r = zsc_state_stats(ctx, &s)) return die("zsc_state_stats", r);
+            stats_line(sf, done, s, inv_vol);
+        }
+        if (done % in.save == 0)
+            if (int r = vtk(ctx, dir, vmode, relax, done)) return die("zsc_vtk_save", r);
+    }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
-    std::string dir = ".";
+    std::string dir = ".", restart_in, restart_out;
     int prec = ZSC_F32;
+    int vmode = 0;  // 0 ASCII (as the reference), 1 binary, -1 none
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "-d") && i + 1 < argc) {
             dir = argv[++i];
+        } else if (!strcmp(argv[i], "-r") && i + 1 < argc) {
+            restart_in = argv[++i];
+        } else if (!strcmp(argv[i], "-s") && i + 1 < argc) {
+            restart_out = argv[++i];
+        } else if (!strcmp(argv[i], "-v") && i + 1 < argc) {
+            const char *v = argv[++i];
+            vmode = !strcmp(v, "ascii") ? 0 : !strcmp(v, "binary") ? 1 : !strcmp(v, "none") ? -1 : -2;
+            if (vmode == -2) {
+                fprintf(stderr, "zsc_run: unknown VTK mode %s\n", v);
+                return 2;
+            }
         } else if (!strcmp(argv[i], "-p") && i + 1 < argc) {
             const char *p = argv[++i];
             prec = !strcmp(p, "f64") ? ZSC_F64 : !strcmp(p, "f32") ? ZSC_F32
@@ -68,7 +122,8 @@ int main(int argc, char **argv)
                 return 2;
             }
         } else {
-            fprintf(stderr, "usage: zsc_run [-d DIR] [-p f64|f32|bf16|fp16]\n");
+            fprintf(stderr, "usage: zsc_run [-d DIR] [-p f64|f32|bf16|fp16] [-v ascii|binary|none] "
+                            "[-r RESTART_IN] [-s RESTART_OUT]\n");
             return 2;
         }
     }
@@ -81,6 +136,7 @@ int main(int argc, char **argv)
     zsc_params p;
     CK(zsc_derive_params(&in, nb > 0 ? bub[3] : 0.0, &p));
     if (in.stat <= 0) in.stat = 1 << 30;
+    if (in.save <= 0) in.save = 1 << 30;
 
     printf("*** Multiphase Zheng-Shu-Chew LBM 3D Simulation\n");
     printf("*** MI355X HIP implementation (libzsc)\n\n");
@@ -98,67 +154,74 @@ int main(int argc, char **argv)
     (void)hipEventCreate(&e2);
     (void)hipEventCreate(&e3);
     (void)hipEventRecord(e0, 0);
-    CK(zsc_state_init_bubbles(ctx, bub.data(), nb));
+    zsc_stats s;
+    int64_t resumed = -1;
+    double laplace[4] = {0, 0, 0, 0};  // Pdif, Perr, Ref, Umax after relaxation
+    double relax_vef = 0.0;
+    if (!restart_in.empty()) {
+        CK(zsc_state_load(ctx, restart_in.c_str(), &resumed));
+        printf("Restarted from %s (step %lld).\n", restart_in.c_str(), (long long)resumed);
+    } else {
+        CK(zsc_state_init_bubbles(ctx, bub.data(), nb));
+    }
     printf("Data initialization completed.\n");
     (void)hipDeviceSynchronize();
     (void)hipEventRecord(e1, 0);
 
-    zsc_stats s;
-    // relaxation: gravity off (src/main.cu:361-474)
-    StatsFile relax{dir + "/relax_stats.out"};
-    {
-        FILE *fp = fopen(relax.path.c_str(), "w");
-        if (fp) {
-            fprintf(fp, "#step\t\tux\t\tuy\t\tuz\t\tUmax\t\tVef\t\tRef\n");
-            fclose(fp);
+    if (resumed < 0) {
+        // relaxation: gravity off (src/main.cu:296-474)
+        StatsFile relax{dir + "/relax_stats.out"};
+        {
+            FILE *fp = fopen(relax.path.c_str(), "w");
+            if (fp) {
+                fprintf(fp, "#step\t\tux\t\tuy\t\tuz\t\tUmax\t\tVef\t\tRef\n");
+                fclose(fp);
+            }
         }
+        CK(zsc_state_stats(ctx, &s));
+        const double inv_relax = s.bubble_volume > 0 ? 1.0 / s.bubble_volume : 0.0;
+        stats_line(relax, 0, s, inv_relax);
+        CK(vtk(ctx, dir, vmode, true, 0));
+        CK(zsc_set_gravity(ctx, 0));
+        if (int r = run_phase(ctx, in, dir, vmode, true, relax, inv_relax, 1, in.relax_step, nullptr))
+            return r;
+        // Laplace-law check of src/logsave.c:155-162 (single static bubble)
+        CK(zsc_state_stats(ctx, &s));
+        const double cells = (double)in.xmax * in.ymax * in.zmax;
+        const double vin = s.bubble_volume, vout = cells - vin;
+        const double pin = vin > 0 ? s.p_in_sum / vin : 0.0, pout = vout > 0 ? s.p_out_sum / vout : 0.0;
+        const double ro = nb > 0 ? bub[3] : 0.0;
+        laplace[0] = pin - pout;
+        laplace[1] = ro > 0 ? 0.5 * (2.0 * in.sigma / ro - laplace[0]) * ro / in.sigma : 0.0;
+        laplace[2] = pow(vin * kInvPi * 0.75, 1.0 / 3.0);
+        laplace[3] = sqrt(s.umax_sq);
+        relax_vef = vin * inv_relax;
+        printf("Relaxation run completed.\n");
     }
-    CK(zsc_state_stats(ctx, &s));
-    const double inv_relax = s.bubble_volume > 0 ? 1.0 / s.bubble_volume : 0.0;
-    stats_line(relax, 0, s, inv_relax);
-    CK(zsc_set_gravity(ctx, 0));
-    for (int step = 1; step <= in.relax_step;) {
-        const int next = std::min(in.relax_step, (step / in.stat + 1) * in.stat);
-        const int chunk = next - step + 1;
-        CK(zsc_step(ctx, chunk));
-        step += chunk;
-        if ((step - 1) % in.stat == 0) {
-            CK(zsc_state_stats(ctx, &s));
-            stats_line(relax, step - 1, s, inv_relax);
-        }
-    }
-    printf("Relaxation run completed.\n");
     CK(zsc_sync(ctx));
     (void)hipEventRecord(e2, 0);
 
-    // evolution: gravity on (src/main.cu:479-610), maxStep-1 steps
+    // evolution: gravity on (src/main.cu:476-610), maxStep-1 steps
     StatsFile evo{dir + "/stats.out"};
+    const int first = resumed < 0 ? 1 : (int)resumed + 1;
     {
-        FILE *fp = fopen(evo.path.c_str(), "w");
+        FILE *fp = fopen(evo.path.c_str(), resumed < 0 ? "w" : "a");
         if (fp) {
-            fprintf(fp, "#step\t\tux\t\tuy\t\tuz\t\tUmax\t\tVef\t\tRef\n");
+            if (resumed < 0) fprintf(fp, "#step\t\tux\t\tuy\t\tuz\t\tUmax\t\tVef\t\tRef\n");
             fclose(fp);
         }
     }
     CK(zsc_state_stats(ctx, &s));
     const double inv_init = s.bubble_volume > 0 ? 1.0 / s.bubble_volume : 0.0;
-    stats_line(evo, 0, s, inv_init);
+    if (resumed < 0) {
+        stats_line(evo, 0, s, inv_init);
+        CK(vtk(ctx, dir, vmode, false, 0));
+    }
     CK(zsc_set_gravity(ctx, 1));
     printf("Starting evolution run ...\n");
     double evo_ms = 0.0;
     const int last = in.max_step - 1;
-    for (int step = 1; step <= last;) {
-        const int next = std::min(last, (step / in.stat + 1) * in.stat);
-        const int chunk = next - step + 1;
-        double ms = 0.0;
-        CK(zsc_step_timed(ctx, chunk, &ms));
-        evo_ms += ms;
-        step += chunk;
-        if ((step - 1) % in.stat == 0) {
-            CK(zsc_state_stats(ctx, &s));
-            stats_line(evo, step - 1, s, inv_init);
-        }
-    }
+    if (int r = run_phase(ctx, in, dir, vmode, false, evo, inv_init, first, last, &evo_ms)) return r;
     printf("Evolution run completed.\n");
     CK(zsc_sync(ctx));
     (void)hipEventRecord(e3, 0);
@@ -168,12 +231,17 @@ int main(int argc, char **argv)
     (void)hipEventElapsedTime(&t_relax, e1, e2);
     (void)hipEventElapsedTime(&t_main, e2, e3);
     CK(zsc_state_stats(ctx, &s));
+    // final configuration, named with the loop counter's end value maxStep
+    // as src/main.cu:628-634 does
+    CK(vtk(ctx, dir, vmode, false, in.max_step));
+    if (!restart_out.empty()) CK(zsc_state_save(ctx, restart_out.c_str(), last));
     const double cells = (double)in.xmax * in.ymax * in.zmax;
-    const double mlups = last > 0 && evo_ms > 0 ? cells * last / (evo_ms * 1e3) : 0.0;
+    const int nrun = last >= first ? last - first + 1 : 0;
+    const double mlups = nrun > 0 && evo_ms > 0 ? cells * nrun / (evo_ms * 1e3) : 0.0;
     printf("Setup time      : %6.3f ms\n", t_setup);
     printf("Relaxation time : %6.3f ms\n", t_relax);
     printf("Evolution time  : %6.3f ms\n", t_main);
-    printf("Time for lb step: %6.3f ms\n", last > 0 ? evo_ms / last : 0.0);
+    printf("Time for lb step: %6.3f ms\n", nrun > 0 ? evo_ms / nrun : 0.0);
     printf("MLUPS (step kernels only): %.1f\n", mlups);
 
     FILE *fp = fopen((dir + "/runlog.out").c_str(), "w");
@@ -193,6 +261,16 @@ int main(int argc, char **argv)
         fprintf(fp, "High Fluid Density = %6.3f\n", in.rho_h);
         fprintf(fp, "Tau (Density)      = %6.3f\n", in.tau_rho);
         fprintf(fp, "Tau (Phase)        = %6.3f\n\n", in.tau_phi);
+        if (resumed < 0) {
+            fprintf(fp, "RELAXATION RESULTS\n");
+            fprintf(fp, "Effective Radius   = %e\n", laplace[2]);
+            fprintf(fp, "Phase Conservation = %e\n", relax_vef);
+            fprintf(fp, "(Pin - Pout)       = %e\n", laplace[0]);
+            fprintf(fp, "Laplace Error      = %e\n", laplace[1]);
+            fprintf(fp, "Parasitic Velocity = %e\n\n", laplace[3]);
+        } else {
+            fprintf(fp, "RESTART\nResumed at step   = %lld\n\n", (long long)resumed);
+        }
         const double vol = s.bubble_volume;
         fprintf(fp, "OUTPUT RESULTS\n");
         fprintf(fp, "Effective Radius   = %e\n", pow(vol * kInvPi * 0.75, 1.0 / 3.0));
@@ -204,7 +282,7 @@ int main(int argc, char **argv)
         fprintf(fp, "Setup Time      = %9.6f\n", t_setup * 1e-3);
         fprintf(fp, "Relaxation Time = %9.6f\n", t_relax * 1e-3);
         fprintf(fp, "Main Loop Time  = %9.6f\n", t_main * 1e-3);
-        fprintf(fp, "LBM step Time   = %9.6f\n", last > 0 ? evo_ms * 1e-3 / last : 0.0);
+        fprintf(fp, "LBM step Time   = %9.6f\n", nrun > 0 ? evo_ms * 1e-3 / nrun : 0.0);
         fprintf(fp, "MLUPS           = %9.6f\n", mlups);
         fclose(fp);
     }

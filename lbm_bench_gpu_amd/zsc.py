@@ -12,6 +12,7 @@ libzsc's HIP kernels.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -261,6 +262,27 @@ class Context:
         s = ZscStats()
         check(lib().zsc_state_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k in STATS_FIELDS}
+
+    def pressure(self) -> np.ndarray:
+        """Pressure of the A state (src/vtksave.c:75-119), FP64 (nz, ny, nx)."""
+        nx, ny, nz = self.dims
+        out = np.empty((nz, ny, nx))
+        check(lib().zsc_state_pressure(self._h, _dptr(out)))
+        return out
+
+    def vtk_save(self, path: str, binary: bool = False):
+        """Legacy VTK file of phi, pressure and velocity (src/vtksave.c)."""
+        check(lib().zsc_vtk_save(self._h, os.fsencode(path), 1 if binary else 0))
+
+    def save(self, path: str, step: int = 0):
+        """Restart file of the state as held; load() continues it bit for bit."""
+        check(lib().zsc_state_save(self._h, os.fsencode(path), int(step)))
+
+    def load(self, path: str) -> int:
+        """Restores state, parameters and gravity switch; returns the step."""
+        st = ctypes.c_int64(0)
+        check(lib().zsc_state_load(self._h, os.fsencode(path), ctypes.byref(st)))
+        return st.value
 
     def state_export(self, f, g):
         check(lib().zsc_state_export(self._h, self._pops(f, 7), self._pops(g, 19)))

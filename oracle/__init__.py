@@ -24,6 +24,10 @@ Pinning of each function (the checks live in ``tests/test_oracle.py``):
                       halo semantics (tests/ghost_ref.py), plus translation
                       covariance and exact conservation.
   zo_stats            brute-force NumPy sums.
+  zo_pressure         bulk value rho c_s^2 at phi = +-phi*; double-well and
+                      square-gradient terms on Fourier modes of phi (closed
+                      forms from the stencil symbols); the pressure sums of
+                      zo_stats.
 """
 from __future__ import annotations
 
@@ -74,9 +78,10 @@ def lib():
         L.zo_stream_g.argtypes = [_I, _I, _I, _P, _P]
         L.zo_run.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef), _I, _I, _P, _P, _P]
         L.zo_stats.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef)] + [_P] * 6
+        L.zo_pressure.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef)] + [_P] * 3
         for f in ("zo_derive", "zo_init_phi", "zo_init_f", "zo_init_g", "zo_update_phi",
                   "zo_update_rho", "zo_update_velocity", "zo_collision_f", "zo_collision_g",
-                  "zo_stream_f", "zo_stream_g", "zo_run", "zo_stats"):
+                  "zo_stream_f", "zo_stream_g", "zo_run", "zo_stats", "zo_pressure"):
             getattr(L, f).restype = None
         _lib = L
     return _lib
@@ -199,6 +204,15 @@ def stream(coef: ZoCoef, fb, gb):
     L.zo_stream_f(nx, ny, nz, ctypes.byref(coef), _p(np.ascontiguousarray(fb)), _p(fa))
     L.zo_stream_g(nx, ny, nz, _p(np.ascontiguousarray(gb)), _p(ga))
     return fa, ga
+
+
+def pressure(coef: ZoCoef, phi, rho) -> np.ndarray:
+    """Pressure field (src/vtksave.c:75-119), FP64, shape of phi."""
+    nz, ny, nx = phi.shape
+    out = np.empty_like(phi, dtype=np.float64)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (phi, rho)]
+    lib().zo_pressure(nx, ny, nz, ctypes.byref(coef), *map(_p, args), _p(out))
+    return out
 
 
 def stats(coef: ZoCoef, phi, rho, ux, uy, uz) -> np.ndarray:

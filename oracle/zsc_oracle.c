@@ -404,6 +404,34 @@ void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
     }
 }
 
+/* Pressure of src/vtksave.c:75-119 (src/logsave.c:84-137 is the same
+ * expression): alpha(phi^2(3phi^2 - 2phi*^2) - phi*^4)
+ * - kappa(phi lap - |grad phi|^2/2) + Cs_sq rho, periodic stencil. */
+static inline double pressure_at(const zo_coef *c, const double p[19], double rho)
+{
+    double phin = p[0], phin2 = phin * phin;
+    double lap = lap19(p);
+    double gx = gx19(p) * INV12, gy = gy19(p) * INV12, gz = gz19(p) * INV12;
+    double gsq = gx * gx + gy * gy + gz * gz;
+    return c->alpha * (phin2 * (3.0 * phin2 - 2.0 * c->phi_star_sq) -
+                       c->phi_star_sq * c->phi_star_sq) -
+           c->kappa * (phin * lap - 0.5 * gsq) + CS_SQ * rho;
+}
+
+void zo_pressure(int nx, int ny, int nz, const zo_coef *c, const double *phi,
+                 const double *rho, double *p_out)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                size_t idx = cell(nx, ny, x, y, z);
+                double p[19];
+                gather19(nx, ny, nz, phi, x, y, z, p);
+                p_out[idx] = pressure_at(c, p, rho[idx]);
+            }
+}
+
 void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
               const double *rho, const double *ux, const double *uy,
               const double *uz, double *out)
@@ -424,14 +452,8 @@ void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
                 if (usq > umax) umax = usq;
                 double p[19];
                 gather19(nx, ny, nz, phi, x, y, z, p);
-                double phin = p[0], phin2 = phin * phin;
-                double lap = lap19(p);
-                double gx = gx19(p) * INV12, gy = gy19(p) * INV12, gz = gz19(p) * INV12;
-                double gsq = gx * gx + gy * gy + gz * gz;
                 /* src/logsave.c:135-137 */
-                double pres = c->alpha * (phin2 * (3.0 * phin2 - 2.0 * c->phi_star_sq) -
-                                          c->phi_star_sq * c->phi_star_sq) -
-                              c->kappa * (phin * lap - 0.5 * gsq) + rho[idx] / 3.0;
+                double pres = pressure_at(c, p, rho[idx]);
                 if (phi[idx] >= 0.0) {
                     vol += 1.0;
                     sx += ux[idx];

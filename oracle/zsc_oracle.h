@@ -82,6 +82,10 @@ void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
  * out[6]  max |u|^2 over i,j in [1,n-2]
  * out[7]  pressure sum inside (phi>=0)   out[8] pressure sum outside
  * out[9]  phi>=0 count over the full box */
+/* Pressure field of src/vtksave.c:75-119 from phi and rho (periodic). */
+void zo_pressure(int nx, int ny, int nz, const zo_coef *c, const double *phi,
+                 const double *rho, double *p_out);
+
 void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
               const double *rho, const double *ux, const double *uy,
               const double *uz, double *out);

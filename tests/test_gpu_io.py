@@ -1,0 +1,279 @@
+"""Pressure, VTK output and restart files of libzsc, through the C-ABI.
+
+* the pressure field (src/vtksave.c:75-119) of an FP64 context equals the
+  oracle's bit for bit, and an FP32 context's within FP32 rounding;
+* an ASCII VTK file of an FP64 context is, line for line, the file the
+  reference's vtkSave layout gives for the oracle's fields ("%f" values);
+  the BINARY form holds the same fields as big-endian float32;
+* a restart file continues a run bit for bit at every precision, whether the
+  state was held post-collision or post-stream, and is refused for another
+  box or precision, or when truncated.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbm_bench_gpu_amd import zsc
+from lbm_bench_gpu_amd._lib import ZSC_ERR_ARG, ZSC_ERR_IO, ZscError
+from tests.test_gpu_parity import oracle_coef, rel, run_oracle, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_pressure_f64_matches_oracle(need_gpu):
+    dims = (20, 14, 12)
+    p, phi0 = small_case(dims, seed=21)
+    st = run_oracle(p, phi0, 6)
+    phi, rho, *_ = st.moments()
+    ref = O.pressure(oracle_coef(p), phi, rho)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(6)
+        got = ctx.pressure()
+    print("pressure f64 rel", rel(got, ref), flush=True)
+    assert rel(got, ref) == 0.0
+
+
+def test_pressure_f32_close_to_oracle_of_its_moments(need_gpu):
+    dims = (16, 16, 10)
+    p, phi0 = small_case(dims, seed=22)
+    with zsc.Context(*dims, precision="f32") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(4)
+        phi, rho, *_ = ctx.moments()
+        got = ctx.pressure()
+    ref = O.pressure(oracle_coef(p), phi, rho)
+    assert rel(got, ref) < 1e-6
+
+
+def _vtk_text(dims, phi, pres, ux, uy, uz):
+    """The layout of src/vtksave.c:44-138 for fields shaped (nz, ny, nx)."""
+    nx, ny, nz = dims
+    out = ["# vtk DataFile Version 2.0", None, "ASCII", "", "DATASET STRUCTURED_POINTS",
+           f"DIMENSIONS {nx} {ny} {nz}", "ORIGIN 1 1 1", "SPACING 1 1 1", "",
+           f"POINT_DATA {nx * ny * nz}", "", "SCALARS Phi float", "LOOKUP_TABLE default"]
+    out += ["%f" % v for v in phi.ravel()]
+    out += ["", "SCALARS Pressure float", "LOOKUP_TABLE default"]
+    out += ["%f" % v for v in pres.ravel()]
+    out += ["", "Vectors Velocity float"]
+    out += ["%f %f %f" % t for t in zip(ux.ravel(), uy.ravel(), uz.ravel())]
+    return out
+
+
+def test_vtk_ascii_matches_reference_layout(need_gpu, tmp_path):
+    dims = (12, 10, 8)
+    p, phi0 = small_case(dims, seed=23)
+    st = run_oracle(p, phi0, 5)
+    phi, rho, ux, uy, uz = st.moments()
+    pres = O.pressure(oracle_coef(p), phi, rho)
+    path = str(tmp_path / "data_00_0000005.vtk")
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(5)
+        ctx.vtk_save(path)
+    lines = open(path).read().split("\n")
+    expect = _vtk_text(dims, phi, pres, ux, uy, uz)
+    assert lines[-1] == "" and len(lines) - 1 == len(expect)
+    bad = [i for i, (a, b) in enumerate(zip(lines, expect)) if b is not None and a != b]
+    assert not bad, [(i, lines[i], expect[i]) for i in bad[:5]]
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_vtk_binary_holds_the_fields(need_gpu, tmp_path, prec):
+    dims = (16, 12, 9)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, seed=24)
+    path = str(tmp_path / "state.vtk")
+    with zsc.Context(*dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(3)
+        phi, rho, ux, uy, uz = ctx.moments()
+        pres = ctx.pressure()
+        ctx.vtk_save(path, binary=True)
+    raw = open(path, "rb").read()
+    assert raw.startswith(b"# vtk DataFile Version 2.0\n")
+    assert b"\nBINARY\n" in raw[:80]
+
+    def block(after: bytes, count: int):
+        i = raw.index(after) + len(after)
+        return np.frombuffer(raw, dtype=">f4", count=count, offset=i).astype(np.float64)
+
+    got_phi = block(b"SCALARS Phi float\nLOOKUP_TABLE default\n", n)
+    got_p = block(b"SCALARS Pressure float\nLOOKUP_TABLE default\n", n)
+    got_u = block(b"Vectors Velocity float\n", 3 * n).reshape(n, 3)
+    f32 = lambda a: a.ravel().astype(np.float32).astype(np.float64)
+    assert np.array_equal(got_phi, f32(phi))
+    assert np.array_equal(got_p, f32(pres))
+    for a, u in enumerate((ux, uy, uz)):
+        assert np.array_equal(got_u[:, a], f32(u))
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16", "fp16"])
+def test_restart_continues_bit_for_bit(need_gpu, tmp_path, prec):
+    dims = (70, 12, 10)
+    p, phi0 = small_case(dims, seed=25)
+    held_a = str(tmp_path / "a.zsc")
+    held_b = str(tmp_path / "b.zsc")
+    with zsc.Context(*dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.set_gravity(False)
+        ctx.save(held_a, step=0)  # post-stream state held (fresh equilibria)
+        ctx.step(6)
+        ctx.save(held_b, step=6)  # post-collision state held
+        ctx.step(5)
+        f_ref, g_ref = ctx.state_get()
+    with zsc.Context(*dims, precision=prec) as ctx:
+        assert ctx.load(held_b) == 6  # parameters and gravity switch come from the file
+        ctx.step(5)
+        f, g = ctx.state_get()
+        assert np.array_equal(f, f_ref) and np.array_equal(g, g_ref)
+        assert ctx.load(held_a) == 0
+        ctx.step(11)
+        f, g = ctx.state_get()
+        assert np.array_equal(f, f_ref) and np.array_equal(g, g_ref)
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16", "fp16"])
+def test_reading_the_state_does_not_change_the_run(need_gpu, prec):
+    """Moments, statistics, pressure and populations are read from a streamed
+    copy; the held post-collision state goes on unchanged."""
+    dims = (40, 16, 12)
+    p, phi0 = small_case(dims, seed=27)
+    out = []
+    for look in (False, True):
+        with zsc.Context(*dims, precision=prec) as ctx:
+            ctx.set_params(p)
+            ctx.state_init(phi0)
+            for _ in range(4):
+                ctx.step(3)
+                if look:
+                    ctx.stats()
+                    ctx.moments()
+                    ctx.pressure()
+                    ctx.state_get()
+            out.append(ctx.state_get())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_restart_refuses_other_box_precision_or_damage(need_gpu, tmp_path):
+    dims = (16, 8, 6)
+    p, phi0 = small_case(dims, seed=26)
+    path = str(tmp_path / "s.zsc")
+    with zsc.Context(*dims, precision="f32") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.save(path)
+    for d, prec in (((16, 8, 7), "f32"), (dims, "f64")):
+        with zsc.Context(*d, precision=prec) as ctx:
+            with pytest.raises(ZscError) as e:
+                ctx.load(path)
+            assert e.value.code == ZSC_ERR_ARG
+    cut = str(tmp_path / "cut.zsc")
+    data = open(path, "rb").read()
+    open(cut, "wb").write(data[:len(data) - 100])
+    with zsc.Context(*dims, precision="f32") as ctx:
+        with pytest.raises(ZscError) as e:
+            ctx.load(cut)
+        assert e.value.code == ZSC_ERR_IO
+        with pytest.raises(ZscError):
+            ctx.step(1)  # a failed load leaves no state
+
+
+PROPS = """maxStep\trelaxStep\tsave\tstat
+{max_step}\t10\t10\t5
+
+xmax\tymax\tzmax
+{n}\t{n}\t{n}
+
+BLOCK_SIZE_X\tBLOCK_SIZE_Y
+32\t8
+
+rhoL\trhoH
+1.0\t10.0
+
+tauRho\ttauPhi
+0.8\t0.7
+
+width\tsigma\tGamma
+4.0\t0.01\t1.0
+
+Eo
+10.0
+
+phiStar 1.0
+gravity 1e-4
+"""
+
+
+def _driver(args, cwd):
+    import subprocess
+    import sys
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "lbm_bench_gpu_amd", "bin", "zsc_run")
+    print("zsc_run", " ".join(args), flush=True)
+    sys.stdout.flush()
+    r = subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def _vtk_phi(path, n):
+    lines = open(path).read().split("\n")
+    i = lines.index("SCALARS Phi float") + 2
+    return np.array([float(v) for v in lines[i:i + n]])
+
+
+def test_driver_run_outputs_and_restart(need_gpu, tmp_path):
+    """zsc_run -p f64 on a 24^3 box: stats/VTK files at the stat and save
+    intervals with the reference's names, the Laplace section of runlog.out,
+    phi of the final VTK file equal to the oracle's after 10 relaxation steps
+    (gravity off) and maxStep-1 evolution steps; then a second run resumed
+    from the first run's restart file reaches the oracle's state further on."""
+    n = 24
+    d1 = tmp_path / "run1"
+    d1.mkdir()
+    (d1 / "properties.in").write_text(PROPS.format(max_step=21, n=n))
+    (d1 / "discrete.in").write_text("nBubbles\n1\n\nxc\tyc\tzc\tRc\n12\t12\t12\t6\n")
+    restart = str(tmp_path / "r.zsc")
+    _driver(["-d", str(d1), "-p", "f64", "-s", restart], str(d1))
+    names = sorted(os.listdir(d1))
+    for f in ["relax_data_00_0000000.vtk", "relax_data_00_0000010.vtk", "data_00_0000000.vtk",
+              "data_00_0000010.vtk", "data_00_0000020.vtk", "data_00_0000021.vtk",
+              "relax_stats.out", "stats.out", "runlog.out"]:
+        assert f in names, (f, names)
+    steps = [int(l.split()[0]) for l in open(d1 / "stats.out") if not l.startswith("#")]
+    assert steps == [0, 5, 10, 15, 20]
+    log = open(d1 / "runlog.out").read()
+    assert "RELAXATION RESULTS" in log and "Laplace Error" in log
+
+    inp = zsc.read_inputs(str(d1 / "properties.in"))
+    b = zsc.read_bubbles(str(d1 / "discrete.in"))
+    p = zsc.derive_params(inp, float(b[0, 3]))
+    phi0 = O.init_phi((n, n, n), b, p.phi_star, p.width)
+    st = O.State(phi0.shape, oracle_coef(p)).init(phi0)
+    st.run(10, grav_on=False)
+    st.run(20, grav_on=True)
+    got = _vtk_phi(d1 / "data_00_0000021.vtk", n ** 3)
+    assert np.abs(got - st.moments()[0].ravel()).max() <= 1e-6  # "%f": 6 decimals
+
+    d2 = tmp_path / "run2"
+    d2.mkdir()
+    (d2 / "properties.in").write_text(PROPS.format(max_step=31, n=n))
+    (d2 / "discrete.in").write_text((d1 / "discrete.in").read_text())
+    _driver(["-d", str(d2), "-p", "f64", "-r", restart, "-v", "binary"], str(d2))
+    assert "relax_stats.out" not in os.listdir(d2)
+    steps = [int(l.split()[0]) for l in open(d2 / "stats.out") if not l.startswith("#")]
+    assert steps == [25, 30]
+    st.run(10, grav_on=True)
+    raw = open(d2 / "data_00_0000031.vtk", "rb").read()
+    key = b"SCALARS Phi float\nLOOKUP_TABLE default\n"
+    got = np.frombuffer(raw, dtype=">f4", count=n ** 3, offset=raw.index(key) + len(key))
+    assert np.array_equal(got, st.moments()[0].ravel().astype(np.float32))

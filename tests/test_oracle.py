@@ -353,3 +353,60 @@ def test_stats_brute_force():
             - c.kappa * (phi * lap_weights(phi) - 0.5 * (gx ** 2 + gy ** 2 + gz ** 2)) + rho / 3)
     assert s[7] == pytest.approx(pres[bub].sum(), rel=1e-10)
     assert s[8] == pytest.approx(pres[inner & (phi < 0)].sum(), rel=1e-10)
+
+
+# -------------------------------------------------------------- pressure --
+def double_well(c, phi):
+    """Bulk free-energy density psi = beta (phi^2 - phi*^2)^2 and its phi-derivative."""
+    d = phi ** 2 - c.phi_star_sq
+    return c.alpha * d * d, 4.0 * c.alpha * phi * d
+
+
+@pytest.mark.parametrize("phi0", [1.0, -1.0, 0.3, -0.7, 0.0])
+def test_pressure_uniform_is_bulk_equation_of_state(phi0):
+    """For uniform phi the stencils vanish and the pressure is the bulk
+    equation of state p0 = phi psi'(phi) - psi(phi) + rho c_s^2; at phi = +-phi*
+    only the ideal part rho c_s^2 remains."""
+    c = O.derive(phys())
+    shape = (4, 5, 6)
+    phi = np.full(shape, phi0)
+    rho = np.random.default_rng(1).uniform(1, 10, shape)
+    psi, dpsi = double_well(c, phi)
+    p = O.pressure(c, phi, rho)
+    assert np.allclose(p, phi * dpsi - psi + rho * CS2, rtol=1e-14, atol=1e-16)
+    if abs(phi0) == 1.0:
+        assert np.allclose(p, rho * CS2, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_pressure_gradient_terms_on_a_fourier_mode(axis):
+    """phi = phi0 + A cos(k x_a): the 19-point Laplacian has symbol 2(cos k - 1)
+    and the isotropic gradient i sin k, so the Korteweg part of the pressure
+    is -kappa (phi lap phi - |grad phi|^2 / 2) with those closed forms."""
+    c = O.derive(phys(sigma=0.05))
+    shape = [8, 8, 8]
+    n = 16
+    shape[2 - axis] = n
+    k = 2 * np.pi * 3 / n
+    coord = np.indices(shape)[2 - axis]
+    phi0, A = 0.2, 0.6
+    phi = phi0 + A * np.cos(k * coord)
+    rho = np.full(shape, 5.5)
+    lap = 2 * (np.cos(k) - 1) * A * np.cos(k * coord)
+    grad = -A * np.sin(k) * np.sin(k * coord)
+    psi, dpsi = double_well(c, phi)
+    expect = phi * dpsi - psi - c.kappa * (phi * lap - 0.5 * grad ** 2) + rho * CS2
+    assert np.allclose(O.pressure(c, phi, rho), expect, rtol=1e-13, atol=1e-15)
+
+
+def test_pressure_sums_of_stats():
+    c = O.derive(phys())
+    shape = (6, 9, 10)
+    st = random_state(shape, 11, c)
+    phi, rho, ux, uy, uz = st.moments()
+    s = O.stats(c, phi, rho, ux, uy, uz)
+    p = O.pressure(c, phi, rho)
+    inner = np.zeros(shape, bool)
+    inner[:, 1:-1, 1:-1] = True
+    assert s[7] == pytest.approx(p[inner & (phi >= 0)].sum(), rel=1e-13)
+    assert s[8] == pytest.approx(p[inner & (phi < 0)].sum(), rel=1e-13)
