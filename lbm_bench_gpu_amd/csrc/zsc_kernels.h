@@ -339,17 +339,6 @@ __device__ inline void pull_g32(const Lattice<S> &B, int64_t n, const NB &nb, co
     }
 }
 
-// Population base address held in scalar registers: the stores then take
-// the scalar-base + 32-bit vector-offset form, instead of a 64-bit vector
-// address chain (one VALU op per population).
-template <class T>
-__device__ inline T *sbase(T *p, int64_t elems)
-{
-    int64_t b = elems * (int64_t)sizeof(T);
-    asm volatile("" : "+s"(b));
-    return (T *)((char *)p + b);
-}
-
 template <class S, class R, int FL>
 __device__ inline void store32(const Lattice<S> &L, int64_t n, uint32_t idx, const Coef<R> &c,
                                const R *f, const R *g)
@@ -357,13 +346,13 @@ __device__ inline void store32(const Lattice<S> &L, int64_t n, uint32_t idx, con
     using SG = typename Traits<S>::SG;
     st32z<FL>(L.f0, idx, f[0]);
 #pragma unroll
-    for (int i = 1; i < 7; ++i) st32z<FL>(sbase(L.f, (int64_t)(i - 1) * n), idx, f[i]);
+    for (int i = 1; i < 7; ++i) st32z<FL>(L.f + (int64_t)(i - 1) * n, idx, f[i]);
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         if constexpr (Codec<SG>::deviation)
-            st32z<FL>(sbase(L.g, (int64_t)i * n), idx, g[i]);  // g holds deviations
+            st32z<FL>(L.g + (int64_t)i * n, idx, g[i]);  // g holds deviations
         else
-            st32<FL>(sbase(L.g, (int64_t)i * n), idx, g[i], g_rest(c, i));
+            st32<FL>(L.g + (int64_t)i * n, idx, g[i], g_rest(c, i));
     }
 }
 
@@ -536,20 +525,28 @@ __global__ void __launch_bounds__(256)
 // and marches a chunk of z planes.  For plane k+1 it pulls the 7 f
 // populations of every tile cell and of a 1-cell ring around the tile,
 // sums them to phi and keeps the plane in an LDS ring of 4 planes; the own
-// cell's pulled f stays in registers for the next iteration.  Plane k is then
+// cell's pulled f waits for the next iteration (in LDS for FP32 compute, in
+// registers for FP64).  Plane k is then
 // collided with its 19-point phi stencil read from LDS and 19 pulled g, and
 // written once, coalesced, to Bout.  Four LDS slots make one barrier per plane
 // sufficient: a wave writing slot (k+2)&3 can only run ahead of a wave that
 // still reads slots k-1..k+1 by one iteration.
 // ---------------------------------------------------------------------------
+// Without the prefetch, the FP32-compute variants fit in 80 vector
+// registers (6 waves per SIMD); the attribute keeps them there.
 template <class S, int TY, int FL, bool PF>
-__global__ void __launch_bounds__(64 * TY)
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF || sizeof(S) == 8 ? 1 : 6)))
     k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
             Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out,
             typename Traits<S>::R *__restrict__ phi_out)
 {
     using R = typename Traits<S>::R;
     __shared__ R sp[4][TY + 2][66];
+    // FP32 compute: each cell's own pulled f of planes k and k+1 wait in LDS
+    // (thread-private slots, no barrier) rather than in 14 vector registers
+    // held across the z loop, which keeps the kernel at 5 waves per SIMD.
+    constexpr bool kStash = sizeof(R) == 4;
+    __shared__ R fsh[kStash ? 2 : 1][kStash ? 7 : 1][kStash ? TY : 1][64];
 
     // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tiles so that neighbouring rows share its L2.
@@ -582,6 +579,10 @@ __global__ void __launch_bounds__(64 * TY)
             else
                 pull_f32<S, R, FL>(in, n, Nbr32(x, y, zc, nx, ny, nz), c, fown);
             sp[slot][ty + 1][tx + 1] = sum_f(fown);
+            if constexpr (kStash) {
+#pragma unroll
+                for (int i = 0; i < 7; ++i) fsh[(zz + 2) & 1][i][ty][tx] = fown[i];
+            }
         }
         for (int h = tid; h < nh; h += 64 * TY) {
             int hx, hy, r, col;
@@ -617,7 +618,7 @@ __global__ void __launch_bounds__(64 * TY)
         }
     };
 
-    R fcur[7], fnext[7];
+    R fcur[7], fnext[kStash ? 1 : 7];
     R gnext[PF ? 19 : 1];
     if constexpr (PF) {
         if (own) {
@@ -627,7 +628,7 @@ __global__ void __launch_bounds__(64 * TY)
                 pull_g32<S, R, FL>(in, n, Nbr32(x, y, z0, nx, ny, nz), c, gnext);
         }
     }
-    plane_phi(z0 - 1, fnext);
+    plane_phi(z0 - 1, fcur);  // phi only; its own f is not collided
     plane_phi(z0, fcur);
     for (int k = z0; k < z1; ++k) {
 #if ZSC_OPAQUE_STRIDE
@@ -661,9 +662,18 @@ __global__ void __launch_bounds__(64 * TY)
                     pull_g32<S, R, FL>(in, n, Nbr32(x, y, k, nx, ny, nz), c, ga);
             }
         }
-        plane_phi(k + 1, fnext);
+        if constexpr (kStash) {
+            R ftmp[7];
+            plane_phi(k + 1, ftmp);
+        } else {
+            plane_phi(k + 1, fnext);
+        }
         __syncthreads();
         if (own) {
+            if constexpr (kStash) {
+#pragma unroll
+                for (int i = 0; i < 7; ++i) fcur[i] = fsh[(k + 2) & 1][i][ty][tx];
+            }
             const int sm = (k + 3) & 3, s0 = k & 3, spl = (k + 1) & 3;
             const int r = ty + 1, q = tx + 1;
             R p[19];
@@ -694,8 +704,10 @@ __global__ void __launch_bounds__(64 * TY)
             store32<S, R, FL>(out, n, idx, c, fb, gb);
             if (phi_out) phi_out[idx] = p[0];  // phi of the post-stream state
         }
+        if constexpr (!kStash) {
 #pragma unroll
-        for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
+            for (int i = 0; i < 7; ++i) fcur[i] = fnext[i];
+        }
     }
 }
 
