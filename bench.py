@@ -45,6 +45,11 @@ def parse():
                     help="steps of the 128^3 adjoint timing (0 disables)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline (0 disables)")
+    ap.add_argument("--workloads", default="1,3",
+                    help="extra workloads reported beside the headline: 1 = 256^3 FP64+FP32 "
+                         "2000 steps, 3 = 1024^3 bf16 500 steps (empty disables)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also time a plain device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -166,6 +171,94 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
             "adjoint_vs_forward": round(adj / fused, 3)}
 
 
+def traffic_from_profile(dims):
+    """HBM bytes per fused launch per precision, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this box (empty when absent)."""
+    nx, ny, nz = dims
+    path = os.path.join(ROOT, "profiles", f"k_fused_{nx}x{ny}x{nz}_traffic.txt")
+    if not os.path.exists(path):
+        return {}
+    out = {}
+    for line in open(path):
+        f = line.split()
+        if len(f) == 2 and not line.startswith("#"):
+            out[f[0]] = float(f[1])
+    return out
+
+
+def copy_probe(mib: int):
+    """A plain device copy (torch copy_ kernel) of a known size: the
+    reference point for the FETCH_SIZE / WRITE_SIZE counter scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    return {"bytes_read": 4 * n, "bytes_written": 4 * n, "ms": round(dt * 1e3, 4),
+            "GB/s": round(8 * n / dt / 1e9, 1)}
+
+
+def workload_1(steps: int):
+    """256^3, 16 bubbles: FP64 and FP32 fused steps (MLUPS, roofline) and the
+    FP32 field error against the FP64 run after `steps` steps."""
+    from lbm_bench_gpu_amd import configs, zsc
+    w = configs.workload(1)
+    p = w.params()
+    out = {"workload": w.name, "steps": steps}
+    fields = {}
+    for prec in ("f64", "f32"):
+        ctx = zsc.Context(*w.dims, precision=prec)
+        try:
+            ctx.set_params(p)
+            ctx.state_init_bubbles(w.bubbles)
+            ctx.step(1)
+            ms = ctx.step_timed(steps - 1)
+            s = ctx.stats()
+            fields[prec] = ctx.moments()
+        finally:
+            ctx.close()
+        per = ms / 1e3 / (steps - 1)
+        achieved = zsc.bytes_per_cell(prec) * w.cells / per / 1e9
+        out[prec] = {"mlups": round(w.cells / per / 1e6, 1), "kernel_ms": round(per * 1e3, 4),
+                     "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "sum_phi": s["sum_phi"]}
+    for name, a, b in zip(("phi", "rho", "ux", "uy", "uz"), fields["f32"], fields["f64"]):
+        out["f32"][f"{name}_max_err"] = float(np.abs(a - b).max())
+    return out
+
+
+def workload_3(steps: int):
+    """1024^3, 512 bubbles, bf16 storage: MLUPS, GB/s and conservation of
+    the phase volume and of total phi over `steps` steps."""
+    from lbm_bench_gpu_amd import configs, zsc
+    w = configs.workload(3)
+    p = w.params()
+    ctx = zsc.Context(*w.dims, precision="bf16")
+    try:
+        ctx.set_params(p)
+        ctx.state_init_bubbles(w.bubbles)
+        s0 = ctx.stats()
+        ctx.step(1)
+        ms = ctx.step_timed(steps - 1)
+        s1 = ctx.stats()
+    finally:
+        ctx.close()
+    per = ms / 1e3 / (steps - 1)
+    gbs = zsc.bytes_per_cell("bf16") * w.cells / per / 1e9
+    return {"workload": w.name, "steps": steps, "precision": "bf16 storage, fp32 compute",
+            "mlups": round(w.cells / per / 1e6, 1), "kernel_ms": round(per * 1e3, 4),
+            "GB/s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "phase_volume_ratio": s1["bubble_volume_full"] / s0["bubble_volume_full"],
+            "sum_phi_drift_rel": (s1["sum_phi"] - s0["sum_phi"]) / abs(s0["sum_phi"]),
+            "sum_rho_drift_rel": (s1["sum_rho"] - s0["sum_rho"]) / abs(s0["sum_rho"])}
+
+
 def cpu_baseline(seconds: float):
     """FP64 oracle on a 128^3 box of the same physics (4 bubbles), timed on the
     host's cores for about `seconds` of work."""
@@ -226,7 +319,21 @@ def main():
     if a.adjoint_steps > 0:
         print("# adjoint", file=sys.stderr, flush=True)
         adj = adjoint_timing(a.adjoint_steps, main_r["mlups"])
+    extra_w = {}
+    for wid in [s for s in a.workloads.split(",") if s]:
+        print(f"# workload {wid}", file=sys.stderr, flush=True)
+        if wid == "1":
+            extra_w[wid] = workload_1(2000)
+        elif wid == "3":
+            extra_w[wid] = workload_3(500)
+        else:
+            raise SystemExit(f"no extra workload {wid}")
+    probe = copy_probe(a.copy_probe) if a.copy_probe > 0 else None
     cpu = cpu_baseline(a.cpu_seconds) if a.cpu_seconds > 0 else None
+    traffic = traffic_from_profile(w.dims)
+    main_r["roofline"]["traffic"] = traffic.get("f32")
+    for k, v in extra.items():
+        v["roofline"]["traffic"] = traffic.get(k)
     out = {
         "metric": "MLUPS + %HBM roofline, FP32 vs bf16-store (err vs FP64); adjoint MLUPS",
         "value": round(main_r["mlups"], 1),
@@ -249,6 +356,8 @@ def main():
                          "roofline": v["roofline"]} for k, v in extra.items()},
         "accuracy_vs_fp64": acc,
         "adjoint": adj,
+        "workloads": extra_w,
+        "copy_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
