@@ -138,7 +138,9 @@ def accuracy(w, params, steps, precs, ty, zchunk, flags):
 def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     """MLUPS of the checkpointed adjoint on the 128^3, 8-bubble workload
     (FP32, checkpoints every 20 steps): cells * steps / time of the backward
-    pass, which includes the recomputation of every segment."""
+    pass including the recomputation of every segment, and of the adjoint
+    steps alone (HIP events around the recomputation and the backward steps
+    of each segment)."""
     import torch
     from lbm_bench_gpu_amd import adjoint, configs, zsc
     w = configs.workload(4)
@@ -146,7 +148,7 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     phi0 = torch.from_numpy(zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width))
     theta = adjoint.theta_of(p)
     tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
-    run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20)
+    run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20, timing=True)
     tgt = run.forward(tt, phi0, steps, keep=False)[:7].sum(0).clone()
     t0 = time.perf_counter()
     a = run.forward(theta, phi0, steps)
@@ -164,11 +166,17 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     ctx.close()
     fused = w.cells * steps / (ms / 1e3) / 1e6
     adj = w.cells * steps / (t3 - t2) / 1e6
+    tm = run.last_timing
+    adj_steps = w.cells * steps / (tm["adjoint_steps_ms"] / 1e3) / 1e6
     return {"workload": w.name, "steps": steps, "checkpoint_every": 20, "precision": "f32",
             "loss": L, "grad": dict(zip(adjoint.THETA, grads)),
             "forward_with_checkpoints_mlups": round(w.cells * steps / (t1 - t0) / 1e6, 1),
             "adjoint_mlups": round(adj, 1), "fused_forward_mlups": round(fused, 1),
-            "adjoint_vs_forward": round(adj / fused, 3)}
+            "adjoint_vs_forward": round(adj / fused, 3),
+            "adjoint_step_mlups": round(adj_steps, 1),
+            "adjoint_step_vs_forward": round(adj_steps / fused, 3),
+            "recompute_ms": round(tm["recompute_ms"], 2),
+            "adjoint_steps_ms": round(tm["adjoint_steps_ms"], 2)}
 
 
 def traffic_from_profile(dims):

@@ -281,8 +281,10 @@ int zsc_adj_backward(zsc_ctx *ctx, const void *state, int pulled, const void *ph
                      const void *lam_next, void *lam_out, void *scratch, double *grads);
 /* With grads == NULL, zsc_adj_backward keeps the step's coefficient sums in
  * the scratch buffer; zsc_adj_flush adds everything kept since the last
- * flush (or zsc_adj_seed) to grads and clears it.  Saves one reduction
- * launch per step. */
+ * flush (or zsc_adj_seed, which drops it) to grads (a no-op when nothing is
+ * kept).  Saves one reduction launch per step.  The sums live in the scratch
+ * buffer: keep passing the same one until the flush; zsc_adj_forward, which
+ * reuses the buffer, fails with ZSC_ERR_STATE while sums are kept. */
 int zsc_adj_flush(zsc_ctx *ctx, void *scratch, double *grads);
 /* grads += derivatives of the initial equilibria (alpha4, kappa) given
  * lam0 = dL/da0. */

@@ -76,7 +76,7 @@ class AdjointRunner:
     """Owns a context and the device buffers of one differentiable run."""
 
     def __init__(self, dims, base: zsc.Params, precision: str = "f32", checkpoint_every: int = 20,
-                 gravity_on: bool = True, device: str = "cuda"):
+                 gravity_on: bool = True, device: str = "cuda", timing: bool = False):
         if precision not in ("f32", "f64"):
             raise ValueError("the adjoint runs at f32 or f64")
         self.dims = tuple(dims)
@@ -98,6 +98,10 @@ class AdjointRunner:
         self.checkpoints = {}
         self.nsteps = 0
         self.phi0 = None
+        # with timing, backward() leaves the device time of the segment
+        # recomputations and of the adjoint steps in last_timing (ms)
+        self.timing = timing
+        self.last_timing = None
 
     def _lattice(self):
         return torch.empty((26, self.n), dtype=self.dtype, device=self.device)
@@ -173,7 +177,12 @@ class AdjointRunner:
         self.grads.zero_()
         lam2 = self._lattice()
         starts = sorted(self.checkpoints)
+        marks = []
         for i in range(len(starts) - 1, -1, -1):
+            if self.timing:
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                ev[0].record(self.stream)
+                marks.append(ev)
             s0 = starts[i]
             s1 = starts[i + 1] if i + 1 < len(starts) else self.nsteps
             # recompute the segment: the input state of every step and phi_n
@@ -193,14 +202,24 @@ class AdjointRunner:
                 states.append(b)
                 phis.append(phi)
                 b = nb
+            if self.timing:
+                marks[-1][1].record(self.stream)
             for n in range(s1 - 1, s0 - 1, -1):
                 j = n - s0
-                check(lib().zsc_adj_backward(h, _vp(states[j]), 0 if n == 0 else 1, _vp(phis[j]),
-                                             _vp(lam), _vp(lam2), _vp(self.scratch), None))
+                check(lib().zsc_adj_backward(h, _vp(states[j]), 0 if n == 0 else 1,
+                                             _vp(phis[j]), _vp(lam), _vp(lam2),
+                                             _vp(self.scratch), None))
                 lam, lam2 = lam2, lam
+            if self.timing:
+                marks[-1][2].record(self.stream)
             del states, phis
         check(lib().zsc_adj_flush(h, _vp(self.scratch), gp))
         check(lib().zsc_adj_init_backward(h, _vp(self.phi0), _vp(lam), _vp(self.scratch), gp))
+        if self.timing:
+            self.stream.synchronize()
+            self.last_timing = {
+                "recompute_ms": sum(m[0].elapsed_time(m[1]) for m in marks),
+                "adjoint_steps_ms": sum(m[1].elapsed_time(m[2]) for m in marks)}
         g = self.grads.cpu().numpy()
         g[14] = loss_slot
         self.lam0 = lam

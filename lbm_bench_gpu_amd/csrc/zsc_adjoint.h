@@ -54,7 +54,7 @@ enum : int {
 template <class R>
 __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const R *ga,
                                 const R *lf, const R *lg, R *afa, R *aga, R &aphi, R &alap, R *agr,
-                                R *fb, double *ap, int grav_on)
+                                R *fb, R *ap, int grav_on)
 {
     // ---- forward recompute
     const R phi = p[0];
@@ -94,7 +94,7 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
         afa[i] = tf * lf[i];
         s_tf += fa[i] * lf[i];
     }
-    ap[kGInvTauPhiOne] += (double)s_tf;
+    ap[kGInvTauPhiOne] += (R)s_tf;
     R a_Af = R(-6) * lf[0];
     for (int i = 1; i < 7; ++i) a_Af += lf[i];
     a_phi += c.inv_tau_phi * lf[0];
@@ -106,8 +106,8 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     }
     a_mu += hg * a_Af;
     a_phi += cfk * a_Cf;
-    ap[kGInvTauPhi] += (double)(phi * lf[0] + R(0.5) * c.gamma * mu * a_Af + c.inv_eta2 * phi * a_Cf);
-    ap[kGInvEta2] += (double)(c.inv_tau_phi * phi * a_Cf);
+    ap[kGInvTauPhi] += (R)(phi * lf[0] + R(0.5) * c.gamma * mu * a_Af + c.inv_eta2 * phi * a_Cf);
+    ap[kGInvEta2] += (R)(c.inv_tau_phi * phi * a_Cf);
 
     // ---- g collision (src/collision_g.cu:228-313), reversed
     const R tg = c.inv_tau_rho_one;
@@ -116,7 +116,7 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
         aga[i] = tg * lg[i];
         s_tg += ga[i] * lg[i];
     }
-    ap[kGInvTauRhoOne] += (double)s_tg;
+    ap[kGInvTauRhoOne] += (R)s_tg;
     const R ut[3] = {R(3) * v[0], R(3) * v[1], R(3) * v[2]};
     const R UF = v[0] * F[0] + v[1] * F[1] + v[2] * F[2];
     const R Q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -162,12 +162,12 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
         a_F[a] += a_S;
         a_F[b] += sg[d] * a_S;
     }
-    ap[kGK0] += (double)g_k0;
-    ap[kGK1] += (double)g_k1;
-    ap[kGK2] += (double)g_k2;
-    ap[kGKC0] += (double)g_kc0;
-    ap[kGKC1] += (double)g_kc1;
-    ap[kGKC2] += (double)g_kc2;
+    ap[kGK0] += (R)g_k0;
+    ap[kGK1] += (R)g_k1;
+    ap[kGK2] += (R)g_k2;
+    ap[kGKC0] += (R)g_kc0;
+    ap[kGKC1] += (R)g_kc1;
+    ap[kGKC2] += (R)g_kc2;
     // X = 3 phi mu + rho (1 - 1.5 Q)
     a_phi += R(3) * mu * a_X;
     a_mu += R(3) * phi * a_X;
@@ -187,11 +187,11 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     // F = mu grad(phi) + grav H(phi) e_z
     a_mu += G[0] * a_F[0] + G[1] * a_F[1] + G[2] * a_F[2];
     for (int a = 0; a < 3; ++a) agr[a] = mu * a_F[a];
-    if (grav_on) ap[kGGrav] += (double)(H * a_F[2]);
+    if (grav_on) ap[kGGrav] += (R)(H * a_F[2]);
     // mu = alpha4 phi (phi^2 - phi*^2) - kappa lap
     a_phi += c.alpha4 * (R(3) * phi * phi - c.phi_star_sq) * a_mu;
-    ap[kGAlpha4] += (double)(phi3 * a_mu);
-    ap[kGKappa] += (double)(-lap * a_mu);
+    ap[kGAlpha4] += (R)(phi3 * a_mu);
+    ap[kGKappa] += (R)(-lap * a_mu);
     alap = -c.kappa * a_mu;
     // u = j / rho, rho = sum g, j = sum c g
     R a_j[3];
@@ -229,19 +229,28 @@ __device__ inline void block_reduce_grads(double *ap, double *partial, bool accu
     }
 }
 
-constexpr int kAdjBlocks = 2048;
+// Blocks of the adjoint kernels: one cell per thread up to 16.7 M cells,
+// then a few per thread (per-thread sums at precision R, block and global
+// sums in FP64).  The partial buffers hold kAdjMaxBlocks rows.
+constexpr int kAdjMaxBlocks = 65536;
+inline int adj_blocks(int64_t n)
+{
+    const int64_t b = (n + 255) / 256;
+    return (int)(b < 1 ? 1 : (b > kAdjMaxBlocks ? kAdjMaxBlocks : b));
+}
 
-// Pass 1: lam_out (local part), the three stencil-adjoint fields and
-// per-block coefficient partials.  A and lam are plain (26, n) lattices.
+// Pass 1: lam_out (local part, including phi's local uses), the stencil-
+// adjoint fields a_lap, a_grad (4 fields) and per-block coefficient
+// partials.  A and lam are plain (26, n) lattices.
 template <class R, bool PULL>
 __global__ void __launch_bounds__(256)
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
                 const R *__restrict__ phi, const R *__restrict__ lam, R *__restrict__ lam_out,
                 R *__restrict__ fields, double *partial, int accumulate)
 {
-    double ap[kNumGrad];
+    R ap[kNumGrad];
 #pragma unroll
-    for (int j = 0; j < kNumGrad; ++j) ap[j] = 0.0;
+    for (int j = 0; j < kNumGrad; ++j) ap[j] = R(0);
     ZSC_GRID_STRIDE(idx, n)
     {
         int x, y, z;
@@ -278,21 +287,23 @@ __global__ void __launch_bounds__(256)
         R deta = R(0);
 #pragma unroll
         for (int i = 1; i < 7; ++i) deta += fb[i] * lfd[i];
-        ap[kGEta] += (double)deta;
+        ap[kGEta] += deta;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) lam_out[(int64_t)i * n + idx] = afa[i];
+        for (int i = 0; i < 7; ++i) lam_out[(int64_t)i * n + idx] = afa[i] + aphi;
 #pragma unroll
         for (int i = 0; i < 19; ++i) lam_out[(int64_t)(7 + i) * n + idx] = aga[i];
-        fields[idx] = aphi;
-        fields[n + idx] = alap;
-        fields[2 * n + idx] = agr[0];
-        fields[3 * n + idx] = agr[1];
-        fields[4 * n + idx] = agr[2];
+        fields[idx] = alap;
+        fields[n + idx] = agr[0];
+        fields[2 * n + idx] = agr[1];
+        fields[3 * n + idx] = agr[2];
     }
-    block_reduce_grads<256>(ap, partial, accumulate != 0);
+    double apd[kNumGrad];
+#pragma unroll
+    for (int j = 0; j < kNumGrad; ++j) apd[j] = (double)ap[j];
+    block_reduce_grads<256>(apd, partial, accumulate != 0);
 }
 
-// Pass 2: a_phi = a_phi_local + lap[a_lap] - div[a_grad] (the transposed
+// Pass 2: a_phi's stencil part lap[a_lap] - div[a_grad] (the transposed
 // stencils), added to the adjoint of every f population.
 template <class R>
 __global__ void __launch_bounds__(256)
@@ -305,13 +316,13 @@ __global__ void __launch_bounds__(256)
         cell_xyz(idx, nx, ny, x, y, z);
         Nbr nb(x, y, z, nx, ny, nz);
         R h[19];
+        gather19(fields, nb, h);
+        R a = lap19(h);
         gather19(fields + n, nb, h);
-        R a = fields[idx] + lap19(h);
-        gather19(fields + 2 * n, nb, h);
         R dv = R(2) * (h[1] - h[2]) + h[7] - h[8] + h[9] - h[10] + h[11] - h[12] + h[13] - h[14];
-        gather19(fields + 3 * n, nb, h);
+        gather19(fields + 2 * n, nb, h);
         dv += R(2) * (h[3] - h[4]) + h[7] - h[8] + h[10] - h[9] + h[15] - h[16] + h[17] - h[18];
-        gather19(fields + 4 * n, nb, h);
+        gather19(fields + 3 * n, nb, h);
         dv += R(2) * (h[5] - h[6]) + h[11] - h[12] + h[14] - h[13] + h[15] - h[16] + h[18] - h[17];
         a -= dv * W<R>::inv12;
 #pragma unroll
@@ -407,70 +418,66 @@ __global__ void __launch_bounds__(256) k_adj_accumulate(int nblocks, const doubl
 
 template <class R>
 struct Adj {
-    // scratch: 5n R (stencil fields) + kAdjBlocks*kNumGrad doubles (partials)
-    // scratch: 5n R (stencil fields) | partials | deferred partials
+    // scratch: 4n R stencil fields | partials | deferred partials
+    static size_t field_bytes(int64_t n) { return align256((size_t)n * 4 * sizeof(R)); }
     static size_t scratch_bytes(int64_t n)
     {
-        return align256((size_t)n * 5 * sizeof(R)) + 2 * (size_t)kAdjBlocks * kNumGrad * sizeof(double);
+        return field_bytes(n) + 2 * (size_t)kAdjMaxBlocks * kNumGrad * sizeof(double);
     }
     static double *partials(const Geo &g, void *scratch, int which)
     {
-        return (double *)((char *)scratch + align256((size_t)g.n * 5 * sizeof(R))) +
-               (size_t)which * kAdjBlocks * kNumGrad;
+        return (double *)((char *)scratch + field_bytes(g.n)) + (size_t)which * kAdjMaxBlocks * kNumGrad;
     }
     // A: the step's input state, or (pulled) the previous post-collision
     // lattice, streamed on the fly.
     // grads == nullptr defers the coefficient sums into per-block partials
-    // kept in the scratch buffer until flush().
+    // kept in the scratch buffer until flush(); defer_first: no deferred
+    // sums exist yet, so the partials are written rather than added to.
     static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
                                const R *A, bool pulled, const R *phi, const R *lam, R *lam_out,
-                               void *scratch, double *grads)
+                               void *scratch, double *grads, bool defer_first)
     {
         R *fields = (R *)scratch;
-        const int acc = grads ? 0 : 1;
-        double *partial = partials(g, scratch, acc);
+        const int acc = grads || defer_first ? 0 : 1;
+        const int nb = adj_blocks(g.n);
+        double *partial = partials(g, scratch, grads ? 0 : 1);
+        const Coef<R> c = make_coef<R>(p, grav_on);
         if (pulled)
-            hipLaunchKernelGGL((k_adj_pass1<R, true>), dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny,
-                               g.nz, g.n, make_coef<R>(p, grav_on), grav_on, A, phi, lam, lam_out,
-                               fields, partial, acc);
+            hipLaunchKernelGGL((k_adj_pass1<R, true>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
+                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc);
         else
-            hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(kAdjBlocks), dim3(256), 0, s, g.nx,
-                               g.ny, g.nz, g.n, make_coef<R>(p, grav_on), grav_on, A, phi, lam,
-                               lam_out, fields, partial, acc);
+            hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
+                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc);
         hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            (const R *)fields, lam_out);
         if (grads)
-            hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial,
-                               grads);
+            hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();
     }
-    // grads += deferred partials; the partials are cleared.
+    // grads += deferred partials.
     static hipError_t flush(const Geo &g, hipStream_t s, void *scratch, double *grads)
     {
-        double *d = partials(g, scratch, 1);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, d, grads);
-        hipLaunchKernelGGL(k_from_f64<double>, grid_for(kAdjBlocks * kNumGrad), dim3(256), 0, s,
-                           (int64_t)kAdjBlocks * kNumGrad, (const double *)nullptr, d);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, adj_blocks(g.n),
+                           partials(g, scratch, 1), grads);
         return hipGetLastError();
     }
     static hipError_t seed(const Geo &g, hipStream_t s, const R *A, const R *tgt, R *lam,
                            void *scratch, double *grads)
     {
         double *partial = partials(g, scratch, 0);
-        hipLaunchKernelGGL(k_from_f64<double>, grid_for(kAdjBlocks * kNumGrad), dim3(256), 0, s,
-                           (int64_t)kAdjBlocks * kNumGrad, (const double *)nullptr,
-                           partials(g, scratch, 1));
-        hipLaunchKernelGGL(k_adj_seed<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.n, A, tgt, lam, partial);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
+        const int nb = adj_blocks(g.n);
+        hipLaunchKernelGGL(k_adj_seed<R>, dim3(nb), dim3(256), 0, s, g.n, A, tgt, lam, partial);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();
     }
     static hipError_t init_backward(const Geo &g, const zsc_params &p, hipStream_t s, const R *phi0,
                                     const R *lam, void *scratch, double *grads)
     {
         double *partial = partials(g, scratch, 0);
-        hipLaunchKernelGGL(k_adj_init<R>, dim3(kAdjBlocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+        const int nb = adj_blocks(g.n);
+        hipLaunchKernelGGL(k_adj_init<R>, dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            make_coef<R>(p, 0), phi0, lam, partial);
-        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, kAdjBlocks, partial, grads);
+        hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();
     }
 };

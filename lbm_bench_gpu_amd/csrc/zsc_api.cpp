@@ -87,6 +87,8 @@ struct zsc_ctx {
     double *stats_buf = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     FusedCfg cfg{4, 16, 0};
+    // deferred coefficient sums (backward calls with grads == NULL) exist
+    bool adj_deferred = false;
 
     void *w(int i) const { return (char *)work + (size_t)i * align256((size_t)geo.n * rsize); }
 };
@@ -913,6 +915,9 @@ extern "C" int zsc_adj_init(zsc_ctx *c, const void *phi0, void *a0)
 
 extern "C" int zsc_adj_forward(zsc_ctx *c, const void *a_in, void *a_out, void *scratch)
 {
+    if (c && c->adj_deferred)
+        return fail(ZSC_ERR_STATE, "zsc_adj_forward: the scratch buffer holds deferred coefficient "
+                                   "sums; call zsc_adj_flush first");
     if (!a_in || !a_out || !scratch || a_in == a_out)
         return fail(ZSC_ERR_ARG, "zsc_adj_forward: null or aliased pointer");
     return adj_call(c, "zsc_adj_forward", [&](auto tag) {
@@ -935,6 +940,7 @@ extern "C" int zsc_adj_seed(zsc_ctx *c, const void *a_n, const void *phi_tgt, vo
     return adj_call(c, "zsc_adj_seed", [&](auto tag) {
         using T = ZSC_T;
         void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
+        c->adj_deferred = false;
         return Adj<T>::seed(c->geo, c->stream, (const T *)a_n, (const T *)phi_tgt, (T *)lam, adj_scr,
                             grads);
     });
@@ -954,19 +960,24 @@ extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const
             if (e != hipSuccess) return e;
             phi_n = phi;
         }
-        return Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)state,
-                                pulled != 0, (const T *)phi_n, (const T *)lam_next, (T *)lam_out,
-                                adj_scr, grads);
+        hipError_t e = Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)state,
+                                        pulled != 0, (const T *)phi_n, (const T *)lam_next,
+                                        (T *)lam_out, adj_scr, grads, !grads && !c->adj_deferred);
+        if (e == hipSuccess && !grads) c->adj_deferred = true;
+        return e;
     });
 }
 
 extern "C" int zsc_adj_flush(zsc_ctx *c, void *scratch, double *grads)
 {
     if (!scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_flush: null pointer");
+    if (c && !c->adj_deferred) return ZSC_OK;
     return adj_call(c, "zsc_adj_flush", [&](auto tag) {
         using T = ZSC_T;
-        return Adj<T>::flush(c->geo, c->stream, (char *)scratch + align256((size_t)c->geo.n * sizeof(T)),
-                             grads);
+        hipError_t e = Adj<T>::flush(c->geo, c->stream,
+                                     (char *)scratch + align256((size_t)c->geo.n * sizeof(T)), grads);
+        if (e == hipSuccess) c->adj_deferred = false;
+        return e;
     });
 }
 

@@ -137,3 +137,49 @@ def test_adjoint_workload_vs_central_differences(need_gpu, precision, tol):
     grads, fd = _workload_fd(precision, True)
     for name, a, r in zip(adjoint.THETA, grads, fd):
         assert abs(a - r) <= tol * abs(r), (name, a, r)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_deferred_coefficient_sums_equal_immediate(need_gpu, precision):
+    """Backward calls with grads = NULL keep the coefficient sums in the
+    scratch buffer, also when it is fresh (no zsc_adj_seed before); one
+    zsc_adj_flush then gives what per-step accumulation gives, and
+    zsc_adj_forward refuses to overwrite the kept sums."""
+    import torch
+    from lbm_bench_gpu_amd._lib import ZSC_ERR_STATE, ZscError
+    dims = (14, 10, 8)
+    p, phi0 = small_problem(dims, seed=3)
+    run = adjoint.AdjointRunner(dims, p, precision=precision, checkpoint_every=4)
+    h = run.ctx._h
+    a = [run.forward(adjoint.theta_of(p), torch.from_numpy(phi0), 2)]
+    for _ in range(3):
+        nxt = run._lattice()
+        torch.cuda.synchronize()
+        check(lib().zsc_adj_forward(h, _vp(a[-1]), _vp(nxt), _vp(run.scratch)))
+        a.append(nxt)
+    run.ctx.sync()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    lam3 = torch.randn((26, run.n), dtype=run.dtype, device="cuda", generator=g)
+    run.scratch.fill_(0x7f)  # garbage where the deferred sums will live
+    gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
+    out = {}
+    for mode in ("deferred", "immediate"):
+        run.grads.zero_()
+        lam, lam2 = lam3.clone(), run._lattice()
+        torch.cuda.synchronize()  # torch ops above ran on torch's stream
+        for n in (2, 1, 0):
+            check(lib().zsc_adj_backward(h, _vp(a[n]), 0, None, _vp(lam), _vp(lam2),
+                                         _vp(run.scratch), None if mode == "deferred" else gp))
+            lam, lam2 = lam2, lam
+        if mode == "deferred":
+            with pytest.raises(ZscError) as e:
+                check(lib().zsc_adj_forward(h, _vp(a[0]), _vp(lam2), _vp(run.scratch)))
+            assert e.value.code == ZSC_ERR_STATE
+            check(lib().zsc_adj_flush(h, _vp(run.scratch), gp))
+        run.ctx.sync()
+        out[mode] = (lam.clone(), run.grads.clone())
+    assert torch.equal(out["deferred"][0], out["immediate"][0])
+    d, i = out["deferred"][1], out["immediate"][1]
+    print("deferred", d[:14].tolist(), "\nimmediate", i[:14].tolist(), flush=True)
+    # the same FP64 sums in another order
+    assert torch.allclose(d, i, rtol=1e-9, atol=1e-12 * float(i.abs().max()))
