@@ -260,3 +260,45 @@ def test_launch_shapes_agree(need_gpu, prec, tol):
         else:
             ef, eg = rel(f, ref[0]), rel(g, ref[1])
             assert ef <= tol and eg <= tol, (ty, zc, fl, ef, eg)
+
+
+@pytest.mark.parametrize("dims,cfg", [
+    ((1, 1, 1), (1, 0, 0)),
+    ((2, 3, 1), (4, 0, 0)),
+    ((1, 5, 4), (2, 3, 0)),
+    ((3, 1, 2), (8, 1, 0)),
+    ((130, 5, 3), (4, 2, 0)),    # three x tiles, the last 2 wide
+    ((66, 9, 7), (8, 16, 8)),    # identity tile order
+])
+def test_fused_f64_degenerate_boxes(need_gpu, dims, cfg):
+    """Extents of 1 and 2 cells, where the periodic wrap folds a stencil onto
+    itself, and x tiles of 2 and 66 cells: bit-identical to the oracle."""
+    p, phi0 = small_case(dims, nb=1, seed=17)
+    st = run_oracle(p, phi0, 7)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.set_fused_config(*cfg)
+        ctx.state_init(phi0)
+        ctx.step(7)
+        assert_fields(ctx, st, 0.0, f"degenerate {dims} cfg={cfg}")
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 3), (5, 4, 1)])
+def test_dropin_f64_degenerate_boxes(need_gpu, dims):
+    import torch
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, nb=1, seed=19)
+    st = run_oracle(p, phi0, 4)
+    f, g, mom = _dropin_tensors(n, torch.float64)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        mom[0].copy_(torch.from_numpy(phi0.ravel()))
+        mom[1].fill_(p.rho_mean)
+        ctx.init_f(mom[0], f)
+        ctx.init_g(mom[0], mom[1], g)
+        reference_sequence(ctx, f, g, mom, 4)
+        ctx.sync()
+        fa = np.stack([t[:n].cpu().numpy() for t in f]).reshape(7, nz, ny, nx)
+        ga = np.stack([t[:n].cpu().numpy() for t in g]).reshape(19, nz, ny, nx)
+    assert rel(fa, st.f) == 0.0 and rel(ga, st.g) == 0.0, (rel(fa, st.f), rel(ga, st.g))
