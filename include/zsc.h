@@ -183,7 +183,10 @@ int zsc_state_init(zsc_ctx *ctx, const double *phi_host,
  * semantics, FP64 tanh), for boxes too large for a host array. */
 int zsc_state_init_bubbles(zsc_ctx *ctx, const double *bubbles, int n);
 /* Advances the state by nsteps reference time steps (update -> collide ->
- * stream each, src/main.cu:496-590) with the current gravity switch. */
+ * stream each, src/main.cu:496-590) with the current gravity switch.  The
+ * fused kernel uses 32-bit byte offsets per population: up to 2^30 cells
+ * (1024^3) at FP32/BF16/FP16 and 2^29 at FP64; larger boxes return
+ * ZSC_ERR_ARG. */
 int zsc_step(zsc_ctx *ctx, int nsteps);
 /* Same, and returns the device time of the launches between two HIP events
  * on the context stream (synchronises). */
