@@ -255,47 +255,53 @@ __global__ void __launch_bounds__(256)
     {
         int x, y, z;
         cell_xyz(idx, nx, ny, x, y, z);
-        Nbr nb(x, y, z, nx, ny, nz);
+        // 32-bit element offsets (every (26, n) row is < 4 GiB, checked at
+        // launch): scalar-base + vector-offset loads, fewer index registers
+        const Nbr32 nb(x, y, z, nx, ny, nz);
+        const uint32_t own = nb.at(0, 0, 0);
         R p[19], fa[7], ga[19], lf[7], lg[19], afa[7], aga[19], fb[7], agr[3], aphi, alap;
-        gather19(phi, nb, p);
+#pragma unroll
+        for (int d = 0; d < 19; ++d) p[d] = ld32z<0>(phi, nb.at(cx(d), cy(d), cz(d)));
         if constexpr (PULL) {
             // A is the post-collision lattice of the previous step: stream it
             const Lattice<R> B = {const_cast<R *>(A), const_cast<R *>(A) + n,
                                   const_cast<R *>(A) + 7 * n};
-            pull_f(B, n, nb, c, fa);
-            pull_g(B, n, nb, c, ga);
+            pull_f32<R, R, 0>(B, n, nb, c, fa);
+            pull_g32<R, R, 0>(B, n, nb, c, ga);
         } else {
 #pragma unroll
-            for (int i = 0; i < 7; ++i) fa[i] = A[(int64_t)i * n + idx];
+            for (int i = 0; i < 7; ++i) fa[i] = ld32z<0>(A + (int64_t)i * n, own);
 #pragma unroll
-            for (int i = 0; i < 19; ++i) ga[i] = A[(int64_t)(7 + i) * n + idx];
+            for (int i = 0; i < 19; ++i) ga[i] = ld32z<0>(A + (int64_t)(7 + i) * n, own);
         }
         // S^T: gather the next adjoint from downwind
         R lfd[7];
-        lf[0] = lam[idx];
+        lf[0] = ld32z<0>(lam, own);
         lfd[0] = R(0);
 #pragma unroll
         for (int i = 1; i < 7; ++i) {
-            const R own = lam[(int64_t)i * n + idx];
-            lfd[i] = lam[(int64_t)i * n + nb.at(cx(i), cy(i), cz(i))];
-            lf[i] = c.eta * lfd[i] + c.eta2 * own;
-            lfd[i] -= own;  // for the eta derivative below
+            const R *li = lam + (int64_t)i * n;
+            const R mine = ld32z<0>(li, own);
+            lfd[i] = ld32z<0>(li, nb.at(cx(i), cy(i), cz(i)));
+            lf[i] = c.eta * lfd[i] + c.eta2 * mine;
+            lfd[i] -= mine;  // for the eta derivative below
         }
 #pragma unroll
-        for (int i = 0; i < 19; ++i) lg[i] = lam[(int64_t)(7 + i) * n + nb.at(cx(i), cy(i), cz(i))];
+        for (int i = 0; i < 19; ++i)
+            lg[i] = ld32z<0>(lam + (int64_t)(7 + i) * n, nb.at(cx(i), cy(i), cz(i)));
         adj_cell(c, p, fa, ga, lf, lg, afa, aga, aphi, alap, agr, fb, ap, grav_on);
         R deta = R(0);
 #pragma unroll
         for (int i = 1; i < 7; ++i) deta += fb[i] * lfd[i];
         ap[kGEta] += deta;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) lam_out[(int64_t)i * n + idx] = afa[i] + aphi;
+        for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, own, afa[i] + aphi);
 #pragma unroll
-        for (int i = 0; i < 19; ++i) lam_out[(int64_t)(7 + i) * n + idx] = aga[i];
-        fields[idx] = alap;
-        fields[n + idx] = agr[0];
-        fields[2 * n + idx] = agr[1];
-        fields[3 * n + idx] = agr[2];
+        for (int i = 0; i < 19; ++i) st32z<0>(lam_out + (int64_t)(7 + i) * n, own, aga[i]);
+        st32z<0>(fields, own, alap);
+        st32z<0>(fields + n, own, agr[0]);
+        st32z<0>(fields + 2 * n, own, agr[1]);
+        st32z<0>(fields + 3 * n, own, agr[2]);
     }
     double apd[kNumGrad];
 #pragma unroll
@@ -437,6 +443,7 @@ struct Adj {
                                const R *A, bool pulled, const R *phi, const R *lam, R *lam_out,
                                void *scratch, double *grads, bool defer_first)
     {
+        if ((uint64_t)g.n * sizeof(R) > ((uint64_t)1 << 32)) return hipErrorInvalidValue;
         R *fields = (R *)scratch;
         const int acc = grads || defer_first ? 0 : 1;
         const int nb = adj_blocks(g.n);
