@@ -244,7 +244,7 @@ int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
  * prefetch the next plane's g populations (software pipelining), 8 no
  * XCD-contiguous tile order, and bits
  * 8..15 a number of KiB (<= 128) of unused LDS per block that limits how
- * many blocks share a CU.  Bits 0-2 only apply to the FP32-compute
+ * many blocks share a CU.  Bits 1-2 only apply to the FP32-compute
  * precisions.  No reference counterpart; a
  * tuning knob of this implementation. */
 int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);

@@ -130,7 +130,7 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     else if (precision == ZSC_FP16)
         c->cfg = {8, 16, kFusedNtStore};
     else
-        c->cfg = {4, 16, 0};
+        c->cfg = {4, 16, kFusedNtStore};
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) {
         if (stream) {

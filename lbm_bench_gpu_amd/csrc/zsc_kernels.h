@@ -894,6 +894,8 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
                                   void *phi)
 {
     if constexpr (sizeof(S) == 8) {
+        // FP64: streaming-store hint only (the other bits tune FP32 compute)
+        if (cfg.flags & kNtStore) return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);
         return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
     } else {
         const int fl = cfg.flags & (kNtStore | kNtLoad);
