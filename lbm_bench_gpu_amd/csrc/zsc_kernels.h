@@ -270,13 +270,6 @@ struct NbrRow {
 };
 
 template <int FL, class S, class R>
-__device__ inline R ld32(const S *base, uint32_t i, R rest)
-{
-    const S *p = (const S *)((const char *)base + i * (uint32_t)sizeof(S));
-    return Codec<S>::dec(ld_raw<FL>(p), rest);
-}
-
-template <int FL, class S, class R>
 __device__ inline void st32(S *base, uint32_t i, R v, R rest)
 {
     S *p = (S *)((char *)base + i * (uint32_t)sizeof(S));
