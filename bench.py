@@ -294,10 +294,63 @@ def cpu_baseline(seconds: float, w):
     dt = time.perf_counter() - t0
     cells = nx * ny * nzs
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    del st
     return {"value": round(cells * k / dt / 1e6, 3), "unit": "MLUPS", "cores": cores,
             "kind": "port",
             "sample": f"FP64 oracle (C, OpenMP, {cores} threads) on a {nx}x{ny}x{nzs} z-slab of "
-                      f"the {w.name} initial state, {k} steps in {dt:.1f} s"}
+                      f"the {w.name} initial state, {k} steps in {dt:.1f} s",
+            "workloads": cpu_side_by_side()}
+
+
+def cpu_side_by_side():
+    """The 64^3 parity box run in full (1000 FP64 steps) by the oracle on the
+    host and by the fused step on the GPU, with their largest difference, and
+    the 256^3 box for 20 oracle steps."""
+    import oracle as O
+    from lbm_bench_gpu_amd import configs, zsc
+    out = {}
+    w0 = configs.workload(0)
+    p = w0.params()
+    c = O.ZoCoef()
+    for k, _ in O.ZoCoef._fields_:
+        setattr(c, k, p.values[k])
+    nx, ny, nz = w0.dims
+    phi0 = O.init_phi((nz, ny, nx), w0.bubbles, p.phi_star, p.width)
+    st = O.State((nz, ny, nx), c).init(phi0)
+    t0 = time.perf_counter()
+    st.run(w0.steps)
+    dt = time.perf_counter() - t0
+    ctx = zsc.Context(*w0.dims, precision="f64")
+    try:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(1)
+        ms = ctx.step_timed(w0.steps - 1)
+        f, g = ctx.state_get()
+    finally:
+        ctx.close()
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+    out["0"] = {"workload": w0.name, "steps": w0.steps, "precision": "f64",
+                "cpu_mlups": round(w0.cells * w0.steps / dt / 1e6, 2),
+                "gpu_mlups": round(w0.cells * (w0.steps - 1) / (ms / 1e3) / 1e6, 1),
+                "max_rel_diff_f": rel(f, st.f), "max_rel_diff_g": rel(g, st.g)}
+    del st
+    w1 = configs.workload(1)
+    p = w1.params()
+    for k, _ in O.ZoCoef._fields_:
+        setattr(c, k, p.values[k])
+    nx, ny, nz = w1.dims
+    phi0 = O.init_phi((nz, ny, nx), w1.bubbles, p.phi_star, p.width)
+    st = O.State((nz, ny, nx), c).init(phi0)
+    st.run(1)
+    t0 = time.perf_counter()
+    st.run(20)
+    dt = time.perf_counter() - t0
+    out["1"] = {"workload": w1.name, "steps": 20, "precision": "f64",
+                "cpu_mlups": round(w1.cells * 20 / dt / 1e6, 2)}
+    return out
 
 
 def main():
