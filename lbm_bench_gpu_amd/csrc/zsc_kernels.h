@@ -621,6 +621,10 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF
                 pull_g32<S, R, FL>(in, n, Nbr32(x, y, z0, nx, ny, nz), c, gnext);
         }
     }
+    // Wave priority: a wave issuing the plane's loads (raised) is preferred
+    // by the CU's arbiter over waves in their collision arithmetic
+    // (default), so loads leave early and more bytes are in flight.
+    __builtin_amdgcn_s_setprio(2);
     plane_phi(z0 - 1, fcur);  // phi only; its own f is not collided
     plane_phi(z0, fcur);
     for (int k = z0; k < z1; ++k) {
@@ -634,6 +638,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF
             n = (int64_t)lo;
         }
 #endif
+        __builtin_amdgcn_s_setprio(2);
         R ga[19];
         const uint32_t idx = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)k);
         if constexpr (PF) {
@@ -662,6 +667,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF
             plane_phi(k + 1, fnext);
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
         if (own) {
             if constexpr (kStash) {
 #pragma unroll
