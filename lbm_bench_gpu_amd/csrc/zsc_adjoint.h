@@ -46,15 +46,18 @@ enum : int {
 };
 
 // Reverse mode of one cell's update + collision.  Inputs: the phi stencil p,
-// the post-stream populations fa/ga and the adjoints lf/lg of the
-// post-collision populations.  Outputs: the local adjoints afa/aga of fa/ga
-// (afa without the phi-stencil coupling), the adjoint of phi(x) through
-// its local uses, of lap(phi)(x) and of grad(phi)(x); fb receives the
-// forward post-collision f (for the eta derivative of the streaming).
+// the post-stream populations fa/ga, the adjoints lf/lg of the
+// post-collision populations and lfd (the streaming's upwind-minus-own
+// adjoint differences).  Outputs: the local adjoints afa/aga of fa/ga (afa
+// without the phi-stencil coupling), the adjoint of phi(x) through its local
+// uses, of lap(phi)(x) and of grad(phi)(x), and deta = sum_i fb_i lfd_i (the
+// eta derivative of the streaming, fb the forward post-collision f).  Each
+// intermediate is formed as late, and dropped as early, as the data flow
+// allows, which keeps the kernel's register count down.
 template <class R>
 __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const R *ga,
-                                const R *lf, const R *lg, R *afa, R *aga, R &aphi, R &alap, R *agr,
-                                R *fb, R *ap, int grav_on)
+                                const R *lf, const R *lfd, const R *lg, R *afa, R *aga, R &aphi,
+                                R &alap, R *agr, R &deta, R *ap, int grav_on)
 {
     // ---- forward recompute
     const R phi = p[0];
@@ -80,20 +83,23 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     const R hg = R(0.5) * c.gamma * c.inv_tau_phi;
     const R cfk = c.inv_tau_phi * c.inv_eta2;
     const R Af = hg * mu, Cf = cfk * phi;
-    fb[0] = c.inv_tau_phi_one * fa[0] - R(6) * Af + c.inv_tau_phi * phi;
-    for (int a = 0; a < 3; ++a) {
-        fb[1 + 2 * a] = c.inv_tau_phi_one * fa[1 + 2 * a] + Af + Cf * v[a];
-        fb[2 + 2 * a] = c.inv_tau_phi_one * fa[2 + 2 * a] + Af - Cf * v[a];
+    {
+        R fb[7];
+        fb[0] = c.inv_tau_phi_one * fa[0] - R(6) * Af + c.inv_tau_phi * phi;
+        for (int a = 0; a < 3; ++a) {
+            fb[1 + 2 * a] = c.inv_tau_phi_one * fa[1 + 2 * a] + Af + Cf * v[a];
+            fb[2 + 2 * a] = c.inv_tau_phi_one * fa[2 + 2 * a] + Af - Cf * v[a];
+        }
+        deta = R(0);
+#pragma unroll
+        for (int i = 1; i < 7; ++i) deta += fb[i] * lfd[i];
     }
 
     // ---- f collision (src/collision_f.cu:105-114), reversed
     R a_phi = R(0), a_mu = R(0), a_rho = R(0), a_v[3] = {0, 0, 0}, a_F[3] = {0, 0, 0};
     const R tf = c.inv_tau_phi_one;
     R s_tf = R(0);
-    for (int i = 0; i < 7; ++i) {
-        afa[i] = tf * lf[i];
-        s_tf += fa[i] * lf[i];
-    }
+    for (int i = 0; i < 7; ++i) s_tf += fa[i] * lf[i];
     ap[kGInvTauPhiOne] += (R)s_tf;
     R a_Af = R(-6) * lf[0];
     for (int i = 1; i < 7; ++i) a_Af += lf[i];
@@ -112,10 +118,7 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     // ---- g collision (src/collision_g.cu:228-313), reversed
     const R tg = c.inv_tau_rho_one;
     R s_tg = R(0);
-    for (int i = 0; i < 19; ++i) {
-        aga[i] = tg * lg[i];
-        s_tg += ga[i] * lg[i];
-    }
+    for (int i = 0; i < 19; ++i) s_tg += ga[i] * lg[i];
     ap[kGInvTauRhoOne] += (R)s_tg;
     const R ut[3] = {R(3) * v[0], R(3) * v[1], R(3) * v[2]};
     const R UF = v[0] * F[0] + v[1] * F[1] + v[2] * F[2];
@@ -200,8 +203,10 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
         a_rho += -u[a] * a_u[a] / rho;
     }
 #pragma unroll
+    for (int i = 0; i < 7; ++i) afa[i] = tf * lf[i];
+#pragma unroll
     for (int i = 0; i < 19; ++i)
-        aga[i] += a_rho + R(cx(i)) * a_j[0] + R(cy(i)) * a_j[1] + R(cz(i)) * a_j[2];
+        aga[i] = tg * lg[i] + (a_rho + R(cx(i)) * a_j[0] + R(cy(i)) * a_j[1] + R(cz(i)) * a_j[2]);
     aphi = a_phi;
 }
 
@@ -259,7 +264,7 @@ __global__ void __launch_bounds__(256)
         // launch): scalar-base + vector-offset loads, fewer index registers
         const Nbr32 nb(x, y, z, nx, ny, nz);
         const uint32_t own = nb.at(0, 0, 0);
-        R p[19], fa[7], ga[19], lf[7], lg[19], afa[7], aga[19], fb[7], agr[3], aphi, alap;
+        R p[19], fa[7], ga[19], lf[7], lg[19], afa[7], aga[19], agr[3], aphi, alap, deta;
 #pragma unroll
         for (int d = 0; d < 19; ++d) p[d] = ld32z<0>(phi, nb.at(cx(d), cy(d), cz(d)));
         if constexpr (PULL) {
@@ -289,10 +294,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 0; i < 19; ++i)
             lg[i] = ld32z<0>(lam + (int64_t)(7 + i) * n, nb.at(cx(i), cy(i), cz(i)));
-        adj_cell(c, p, fa, ga, lf, lg, afa, aga, aphi, alap, agr, fb, ap, grav_on);
-        R deta = R(0);
-#pragma unroll
-        for (int i = 1; i < 7; ++i) deta += fb[i] * lfd[i];
+        adj_cell(c, p, fa, ga, lf, lfd, lg, afa, aga, aphi, alap, agr, deta, ap, grav_on);
         ap[kGEta] += deta;
 #pragma unroll
         for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, own, afa[i] + aphi);
