@@ -45,9 +45,11 @@ def parse():
                     help="steps of the 128^3 adjoint timing (0 disables)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline (0 disables)")
-    ap.add_argument("--workloads", default="1,3",
+    ap.add_argument("--workloads", default="1,3,dropin",
                     help="extra workloads reported beside the headline: 1 = 256^3 FP64+FP32 "
-                         "2000 steps, 3 = 1024^3 bf16 500 steps (empty disables)")
+                         "2000 steps, 3 = 1024^3 bf16 500 steps, dropin = the reference's "
+                         "7-kernel step through the drop-in entry points at 256^3 FP32 "
+                         "(empty disables)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
@@ -243,6 +245,53 @@ def workload_1(steps: int):
     return out
 
 
+def dropin_sequence(steps: int):
+    """The reference's per-step launch sequence (update_phi, update_rho,
+    update_velocity, collision_f, collision_g, stream_f, stream_g;
+    src/main.cu:498-566) through the drop-in entry points, FP32, on the 256^3
+    box: MLUPS over the wall time of `steps` steps, beside the fused step."""
+    import torch
+    from lbm_bench_gpu_amd import configs, zsc
+    w = configs.workload(1)
+    p = w.params()
+    n = w.cells
+    dev = "cuda"
+    f = [torch.zeros(2 * n, dtype=torch.float32, device=dev) for _ in range(7)]
+    g = [torch.zeros(2 * n, dtype=torch.float32, device=dev) for _ in range(19)]
+    phi, rho, ux, uy, uz = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(5)]
+    phi.copy_(torch.from_numpy(
+        zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width).ravel()).float())
+    rho.fill_(p.rho_mean)
+    torch.cuda.synchronize()
+    ctx = zsc.Context(*w.dims, precision="f32")
+    try:
+        ctx.set_params(p)
+        ctx.init_f(phi, f)
+        ctx.init_g(phi, rho, g)
+
+        def step():
+            ctx.update_phi(phi, f)
+            ctx.update_rho(rho, g)
+            ctx.update_velocity(rho, ux, uy, uz, g)
+            ctx.collision_f(phi, rho, ux, uy, uz, f)
+            ctx.collision_g(phi, rho, ux, uy, uz, g)
+            ctx.stream_f(f)
+            ctx.stream_g(g)
+        for _ in range(3):
+            step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ctx.sync()
+        dt = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    return {"workload": w.name, "steps": steps, "precision": "f32",
+            "kernels_per_step": 7, "mlups": round(n * steps / dt / 1e6, 1),
+            "ms_per_step": round(dt * 1e3 / steps, 4)}
+
+
 def workload_3(steps: int):
     """1024^3, 512 bubbles, bf16 storage: MLUPS, GB/s and conservation of
     the phase volume and of total phi over `steps` steps."""
@@ -394,6 +443,8 @@ def main():
             extra_w[wid] = workload_1(2000)
         elif wid == "3":
             extra_w[wid] = workload_3(500)
+        elif wid == "dropin":
+            extra_w[wid] = dropin_sequence(200)
         else:
             raise SystemExit(f"no extra workload {wid}")
     probe = copy_probe(a.copy_probe) if a.copy_probe > 0 else None
