@@ -50,8 +50,9 @@ def parse():
                          "2000 steps, 3 = 1024^3 bf16 500 steps, dropin = the reference's "
                          "7-kernel step through the drop-in entry points at 256^3 FP32 "
                          "(empty disables)")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="also time a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--copy-probe", type=int, default=4096, metavar="MIB",
+                    help="time a plain device copy of MIB MiB: the practical read+write "
+                         "ceiling reported beside the roofline (0 disables)")
     return ap.parse_args()
 
 
@@ -453,6 +454,10 @@ def main():
     main_r["roofline"]["traffic"] = traffic.get("f32")
     for k, v in extra.items():
         v["roofline"]["traffic"] = traffic.get(k)
+    if probe:
+        for r in [main_r] + list(extra.values()):
+            r["roofline"]["copy_GBs"] = probe["GB/s"]
+            r["roofline"]["frac_of_copy"] = round(r["roofline"]["achieved"] / probe["GB/s"], 4)
     out = {
         "metric": "MLUPS + %HBM roofline, FP32 vs bf16-store (err vs FP64); adjoint MLUPS",
         "value": round(main_r["mlups"], 1),
