@@ -1,0 +1,76 @@
+"""Error handling of the C-ABI on a live device: calls out of order, at the
+wrong precision, with bad arguments, and a diverged state.  Every call
+returns a status (no abort) and leaves the context usable."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from lbm_bench_gpu_amd import zsc
+from lbm_bench_gpu_amd._lib import (ZSC_ERR_ARG, ZSC_ERR_DIVERGED, ZSC_ERR_PRECISION,
+                                    ZSC_ERR_STATE, ZscError, lib)
+from tests.test_gpu_parity import small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _code(fn, *a):
+    with pytest.raises(ZscError) as e:
+        fn(*a)
+    return e.value.code
+
+
+def test_calls_out_of_order(need_gpu):
+    dims = (16, 8, 6)
+    p, phi0 = small_case(dims, seed=31)
+    with zsc.Context(*dims, precision="f32") as ctx:
+        assert _code(ctx.state_init, phi0) == ZSC_ERR_STATE      # no parameters yet
+        ctx.set_params(p)
+        assert _code(ctx.step, 1) == ZSC_ERR_STATE                # no state yet
+        assert _code(ctx.stats) == ZSC_ERR_STATE
+        assert _code(ctx.state_get) == ZSC_ERR_STATE
+        assert _code(ctx.step, -1) == ZSC_ERR_ARG
+        ctx.state_init(phi0)
+        ctx.step(2)                                               # still usable
+        assert np.isfinite(ctx.stats()["sum_phi"])
+
+
+def test_dropin_and_adjoint_need_f32_or_f64(need_gpu):
+    import torch
+    dims = (8, 8, 4)
+    n = 8 * 8 * 4
+    p, _ = small_case(dims, seed=32)
+    f = [torch.zeros(2 * n, dtype=torch.float32, device="cuda") for _ in range(7)]
+    phi = torch.zeros(n, dtype=torch.float32, device="cuda")
+    for prec in ("bf16", "fp16"):
+        with zsc.Context(*dims, precision=prec) as ctx:
+            ctx.set_params(p)
+            assert _code(ctx.init_f, phi, f) == ZSC_ERR_PRECISION
+            assert _code(ctx.update_phi, phi, f) == ZSC_ERR_PRECISION
+            buf = torch.zeros(26 * n, dtype=torch.float32, device="cuda")
+            rc = lib().zsc_adj_init(ctx._h, ctypes.c_void_p(phi.data_ptr()),
+                                    ctypes.c_void_p(buf.data_ptr()))
+            assert rc == ZSC_ERR_PRECISION
+
+
+def test_bad_launch_shape_is_refused(need_gpu):
+    with zsc.Context(8, 8, 8, precision="f32") as ctx:
+        assert _code(ctx.set_fused_config, 3, 16, 1) == ZSC_ERR_ARG   # ty not 1, 2, 4, 8
+        assert _code(ctx.set_fused_config, 4, -1, 1) == ZSC_ERR_ARG
+        assert _code(ctx.set_fused_config, 4, 16, 200 << 8) == ZSC_ERR_ARG  # LDS pad > 128 KiB
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+def test_diverged_state_is_reported(need_gpu, prec):
+    dims = (8, 6, 4)
+    p, phi0 = small_case(dims, seed=33)
+    with zsc.Context(*dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        f, g = ctx.state_get()
+        g[3, 1, 2, 3] = np.nan
+        ctx.state_set(f, g)
+        assert _code(ctx.stats) == ZSC_ERR_DIVERGED
+        ctx.state_init(phi0)                                      # recovers
+        ctx.step(1)
+        assert np.isfinite(ctx.stats()["sum_rho"])
