@@ -455,8 +455,6 @@ static int step_impl(zsc_ctx *c, int nsteps)
 {
     if (int r = need_ctx(c)) return r;
     if (nsteps < 0) return fail(ZSC_ERR_ARG, "zsc_step: negative step count");
-    if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "zsc_step: no state");
-    if (nsteps == 0) return ZSC_OK;
     // the fused kernel addresses each population array with 32-bit byte
     // offsets (f_0 is 4 or 8 bytes per cell at every precision)
     const uint64_t f0_bytes = (uint64_t)c->geo.n * (c->prec == ZSC_F64 ? 8 : 4);
@@ -465,6 +463,8 @@ static int step_impl(zsc_ctx *c, int nsteps)
                     "zsc_step: %lld cells exceed the fused step's 32-bit population offsets "
                     "(at most %lld cells at this precision); the drop-in kernels take any size",
                     (long long)c->geo.n, (long long)(((uint64_t)1 << 32) / (c->prec == ZSC_F64 ? 8 : 4)));
+    if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "zsc_step: no state");
+    if (nsteps == 0) return ZSC_OK;
     return dispatch(c->prec, [&](auto tag) {
         using S = ZSC_T;
         int left = nsteps;

@@ -302,3 +302,28 @@ def test_dropin_f64_degenerate_boxes(need_gpu, dims):
         fa = np.stack([t[:n].cpu().numpy() for t in f]).reshape(7, nz, ny, nx)
         ga = np.stack([t[:n].cpu().numpy() for t in g]).reshape(19, nz, ny, nx)
     assert rel(fa, st.f) == 0.0 and rel(ga, st.g) == 0.0, (rel(fa, st.f), rel(ga, st.g))
+
+
+def test_fp32_at_the_offset_limit(need_gpu):
+    """1024^3 FP32 (2^30 cells: the last byte offset of a population array
+    is 2^32 - 4, the fused kernel's 32-bit limit; 245 GB of lattices and
+    work arrays): three steps conserve total phi and mass to FP32 round-off,
+    and one cell more in a box is refused."""
+    w = configs.workload(3)
+    p = w.params()
+    with zsc.Context(*w.dims, precision="f32") as ctx:
+        ctx.set_params(p)
+        ctx.state_init_bubbles(w.bubbles)
+        s0 = ctx.stats()
+        ctx.step(3)
+        s1 = ctx.stats()
+    print("1024^3 f32", s0["sum_phi"], s1["sum_phi"], s0["sum_rho"], s1["sum_rho"], flush=True)
+    assert abs(s1["sum_phi"] - s0["sum_phi"]) <= 1e-6 * abs(s0["sum_phi"])
+    assert abs(s1["sum_rho"] - s0["sum_rho"]) <= 1e-6 * abs(s0["sum_rho"])
+    assert abs(s1["bubble_volume_full"] / s0["bubble_volume_full"] - 1) < 1e-3
+    from lbm_bench_gpu_amd._lib import ZSC_ERR_ARG, ZscError
+    with zsc.Context(1024, 1024, 1025, precision="bf16") as ctx:
+        ctx.set_params(p)
+        with pytest.raises(ZscError) as e:
+            ctx.step(1)  # refused before any allocation or state is needed
+        assert e.value.code in (ZSC_ERR_ARG,)
