@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF
     __shared__ R sp[4][TY + 2][66];
     // FP32 compute: each cell's own pulled f of planes k and k+1 wait in LDS
     // (thread-private slots, no barrier) rather than in 14 vector registers
-    // held across the z loop, which keeps the kernel at 5 waves per SIMD.
+    // held across the z loop; this is what lets the kernel fit in 80.
     constexpr bool kStash = sizeof(R) == 4;
     __shared__ R fsh[kStash ? 2 : 1][kStash ? 7 : 1][kStash ? TY : 1][64];
 
