@@ -121,10 +121,11 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     c->prec = precision;
     c->rsize = precision == ZSC_F64 ? 8 : 4;
     // fused-kernel launch shape per precision, from 512^3 sweeps on MI355X:
-    // FP32 gains from the register-held g prefetch; the 16-bit lattices,
-    // which are bound by memory-level parallelism, from 6 waves per SIMD
+    // FP32 gains from the register-held g prefetch (2 blocks per CU, 64
+    // planes); the 16-bit lattices, which are bound by memory-level
+    // parallelism, from 6 waves per SIMD (3 blocks per CU)
     if (precision == ZSC_F32)
-        c->cfg = {8, 16, kFusedNtStore | kFusedPrefetch};
+        c->cfg = {8, 64, kFusedNtStore | kFusedPrefetch};
     else if (precision == ZSC_BF16)
         c->cfg = {8, 32, kFusedNtStore};
     else if (precision == ZSC_FP16)

@@ -317,19 +317,30 @@ __device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const NB &nb, co
 
 // Pulled g of one cell; for deviation storage the values stay deviations
 // (the fused kernel collides them in that form).
-template <class S, class R, int FL, class NB>
+template <class S, class R, int FL, class NB, int LO = 0>
 __device__ inline void pull_g32(const Lattice<S> &B, int64_t n, const NB &nb, const Coef<R> &c,
                                 R *g)
 {
     using SG = typename Traits<S>::SG;
 #pragma unroll
-    for (int i = 0; i < 19; ++i) {
+    for (int i = LO; i < 19; ++i) {
         const SG v = ldx<FL>(B.g + (int64_t)i * n, nb.off(-cx(i), -cy(i), -cz(i)), NB::xstep(-cx(i)));
         if constexpr (Codec<SG>::deviation)
             g[i] = Codec<SG>::dec0(v);
         else
             g[i] = Codec<SG>::dec(v, g_rest(c, i));
     }
+}
+
+// The 19 pulled g of one cell as stored (no decode), so that a load issued
+// a plane ahead is not waited for until its value is used.
+template <class S, int FL, int HI = 19, class NB>
+__device__ inline void pull_g32_raw(const Lattice<S> &B, int64_t n, const NB &nb,
+                                    typename Traits<S>::SG *g)
+{
+#pragma unroll
+    for (int i = 0; i < HI; ++i)
+        g[i] = ldx<FL>(B.g + (int64_t)i * n, nb.off(-cx(i), -cy(i), -cz(i)), NB::xstep(-cx(i)));
 }
 
 template <class S, class R, int FL>
@@ -527,8 +538,28 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Without the prefetch, the FP32-compute variants fit in 80 vector
 // registers (6 waves per SIMD); the attribute keeps them there.
+//
+// Prefetch (PF): g populations 0..NP-1 of plane k+1 are loaded during plane
+// k, issued before plane k+1's f so that both sets of loads are in flight
+// together; the rest load in their own plane.  FP32 storage prefetches 6
+// within a 96-register budget (2 blocks of 512 per CU, like the 117
+// registers of a full prefetch, but faster: 5.55 against 5.70 ms per 512^3
+// step); 16-bit storage prefetches all 19 as raw words.
+template <class S>
+constexpr int prefetch_pops()
+{
+    return sizeof(typename Traits<S>::SG) == 4 ? 6 : 19;
+}
+template <class S, bool PF>
+constexpr int fused_waves()
+{
+    if constexpr (sizeof(S) == 8) return 1;
+    if constexpr (!PF) return 6;
+    return prefetch_pops<S>() < 19 ? 5 : 1;
+}
+
 template <class S, int TY, int FL, bool PF>
-__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF || sizeof(S) == 8 ? 1 : 6)))
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fused_waves<S, PF>())))
     k_fused(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
             Coef<typename Traits<S>::R> c, Lattice<S> in, Lattice<S> out,
             typename Traits<S>::R *__restrict__ phi_out)
@@ -612,14 +643,20 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF
     };
 
     R fcur[7], fnext[kStash ? 1 : 7];
-    R gnext[PF ? 19 : 1];
+    // Prefetched g of the next plane, held as stored: a decode right after
+    // the loads (a shift or conversion for 16-bit storage) would make the
+    // wave wait for them in the plane that issues them.
+    using SG = typename Traits<S>::SG;
+    constexpr int NP = PF ? prefetch_pops<S>() : 0;
+    SG gnext[NP > 0 ? NP : 1];
+    auto prefetch = [&](int zz) {
+        if (xint)
+            pull_g32_raw<S, FL, NP>(in, n, NbrRow(x, y, zz, nx, ny, nz), gnext);
+        else
+            pull_g32_raw<S, FL, NP>(in, n, Nbr32(x, y, zz, nx, ny, nz), gnext);
+    };
     if constexpr (PF) {
-        if (own) {
-            if (xint)
-                pull_g32<S, R, FL>(in, n, NbrRow(x, y, z0, nx, ny, nz), c, gnext);
-            else
-                pull_g32<S, R, FL>(in, n, Nbr32(x, y, z0, nx, ny, nz), c, gnext);
-        }
+        if (own) prefetch(z0);
     }
     // Wave priority: a wave issuing the plane's loads (raised) is preferred
     // by the CU's arbiter over waves in their collision arithmetic
@@ -642,16 +679,25 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(PF
         R ga[19];
         const uint32_t idx = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)k);
         if constexpr (PF) {
-            // software pipeline: plane k's g arrived during the previous
-            // iteration; issue plane k+1's now, to land during this one
+            // software pipeline: plane k's first NP g arrived during the
+            // previous iteration; load the rest of plane k, then issue plane
+            // k+1's first NP, to land during this one
 #pragma unroll
-            for (int i = 0; i < 19; ++i) ga[i] = gnext[i];
-            if (own && k + 1 < z1) {
-                if (xint)
-                    pull_g32<S, R, FL>(in, n, NbrRow(x, y, k + 1, nx, ny, nz), c, gnext);
+            for (int i = 0; i < NP; ++i) {
+                if constexpr (Codec<SG>::deviation)
+                    ga[i] = Codec<SG>::dec0(gnext[i]);
                 else
-                    pull_g32<S, R, FL>(in, n, Nbr32(x, y, k + 1, nx, ny, nz), c, gnext);
+                    ga[i] = Codec<SG>::dec(gnext[i], g_rest(c, i));
             }
+            if constexpr (NP < 19) {
+                if (own) {
+                    if (xint)
+                        pull_g32<S, R, FL, NbrRow, NP>(in, n, NbrRow(x, y, k, nx, ny, nz), c, ga);
+                    else
+                        pull_g32<S, R, FL, Nbr32, NP>(in, n, Nbr32(x, y, k, nx, ny, nz), c, ga);
+                }
+            }
+            if (own && k + 1 < z1) prefetch(k + 1);
         } else {
             if (own) {
                 if (xint)
