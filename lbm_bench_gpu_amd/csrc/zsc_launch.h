@@ -40,7 +40,7 @@ struct LatLayout {
 enum : int {
     kFusedNtStore = 1,      // non-temporal stores of the output lattice
     kFusedNtLoad = 2,       // non-temporal loads (only together with kFusedNtStore)
-    kFusedPrefetch = 4,     // load the next plane's g during this plane
+    kFusedPrefetch = 4,     // load (part of) the next plane's g during this plane
     kFusedIdentityMap = 8,  // no XCD-contiguous tile remapping
 };
 struct FusedCfg {

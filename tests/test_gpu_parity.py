@@ -246,7 +246,8 @@ def test_launch_shapes_agree(need_gpu, prec, tol):
     16-bit store can turn into one 16-bit step: max |diff| <= tol * max |f|."""
     dims = (70, 18, 12)
     p, phi0 = small_case(dims, seed=13)
-    shapes = [(4, 16, 0), (8, 16, 1), (2, 3, 5), (1, 0, 4), (4, 5, 1 | (16 << 8)), (8, 2, 8)]
+    shapes = [(4, 16, 0), (8, 16, 1), (2, 3, 5), (1, 0, 4), (4, 5, 1 | (16 << 8)), (8, 2, 8),
+              (8, 64, 5)]
     ref = None
     for ty, zc, fl in shapes:
         with zsc.Context(*dims, precision=prec) as ctx:
