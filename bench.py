@@ -99,8 +99,9 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags):
 
 def accuracy(w, params, steps, precs, ty, zchunk, flags):
     """max and RMS error of phi, rho, |u| after `steps` steps of each precision
-    against an FP64 run of the same box on the same GPU, and the drift of
-    total phi (FP64 device reductions)."""
+    against an FP64 run of the same box on the same GPU, the drift of total
+    phi (FP64 device reductions) and the ratio of the phase volume (cells with
+    phi >= 0) after and before, which the FP64 run gives for comparison."""
     from lbm_bench_gpu_amd import zsc
 
     def run(prec):
@@ -110,18 +111,20 @@ def accuracy(w, params, steps, precs, ty, zchunk, flags):
                 configure(ctx, ty, zchunk, flags)
             ctx.set_params(params)
             ctx.state_init_bubbles(w.bubbles)
-            s0 = ctx.stats()["sum_phi"]
+            t0 = ctx.stats()
             ctx.step(steps)
-            s1 = ctx.stats()["sum_phi"]
+            t1 = ctx.stats()
             phi, rho, ux, uy, uz = ctx.moments()
         finally:
             ctx.close()
         umag = np.sqrt(ux * ux + uy * uy + uz * uz)
         del ux, uy, uz
-        return phi, rho, umag, s0, s1
+        return (phi, rho, umag, t0["sum_phi"], t1["sum_phi"],
+                t1["bubble_volume_full"] / t0["bubble_volume_full"])
 
     ref = run("f64")
-    out = {"steps": steps, "f64_sum_phi_drift": ref[4] - ref[3]}
+    out = {"steps": steps, "f64_sum_phi_drift": ref[4] - ref[3],
+           "f64_phase_volume_ratio": ref[5]}
     for prec in precs:
         print(f"# accuracy {prec}", file=sys.stderr, flush=True)
         r = run(prec)
@@ -133,6 +136,7 @@ def accuracy(w, params, steps, precs, ty, zchunk, flags):
             del d
         e["sum_phi_drift"] = r[4] - r[3]
         e["sum_phi_drift_rel"] = (r[4] - r[3]) / abs(r[3])
+        e["phase_volume_ratio"] = r[5]
         out[prec] = e
         del r
     return out
