@@ -1,0 +1,84 @@
+"""The benchmark boxes at their full sizes, through properties that do not
+depend on size.
+
+* 256^3, 16 bubbles (seed 1): FP64 fused steps equal the FP64 oracle bit for
+  bit, as on the small parity boxes.
+* 512^3, 128 bubbles (seed 2), 1000 steps: FP32 compute over FP32, bf16 and
+  fp16 storage against an FP64 run of the same box on the GPU (the FP64 path
+  is bit-identical to the oracle, which would take about half an hour for
+  this box on the host): max |dphi| within the stated bounds, 1e-3 for bf16,
+  and total phi and mass conserved to round-off (3e-6 relative: with FP32
+  storage the bulk cells, all alike, round their over-relaxed g update the
+  same way, so the mass error adds up to 1.6e-6 before the rounded map
+  settles; FP64 conserves to 1e-15).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from lbm_bench_gpu_amd import configs, zsc
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_coef(p: zsc.Params):
+    c = O.ZoCoef()
+    for k, _ in O.ZoCoef._fields_:
+        setattr(c, k, p.values[k])
+    return c
+
+
+def test_256_box_fp64_bitwise_vs_oracle(need_gpu):
+    w = configs.workload(1)
+    p = w.params()
+    nx, ny, nz = w.dims
+    phi0 = zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width)
+    st = O.State((nz, ny, nx), oracle_coef(p)).init(phi0)
+    st.run(2)
+    with zsc.Context(*w.dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(2)
+        f, g = ctx.state_get()
+    df = float(np.abs(f - st.f).max())
+    dg = float(np.abs(g - st.g).max())
+    print("256^3 f64 vs oracle: max|df|", df, "max|dg|", dg, flush=True)
+    assert df == 0.0 and dg == 0.0
+
+
+STEPS_512 = 1000
+
+
+@pytest.fixture(scope="module")
+def ref_512(need_gpu):
+    w = configs.workload(2)
+    p = w.params()
+    with zsc.Context(*w.dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init_bubbles(w.bubbles)
+        s0 = ctx.stats()
+        ctx.step(STEPS_512)
+        s1 = ctx.stats()
+        phi = ctx.moments()[0]
+    return w, p, phi, s0, s1
+
+
+@pytest.mark.parametrize("prec,tol_phi,tol_sum", [("f32", 1e-3, 3e-6), ("bf16", 1e-3, 3e-6),
+                                                  ("fp16", 1e-4, 3e-6)])
+def test_512_box_reduced_precision_vs_fp64(ref_512, prec, tol_phi, tol_sum):
+    w, p, phi_ref, r0, r1 = ref_512
+    with zsc.Context(*w.dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        ctx.state_init_bubbles(w.bubbles)
+        s0 = ctx.stats()
+        ctx.step(STEPS_512)
+        s1 = ctx.stats()
+        phi = ctx.moments()[0]
+    e_phi = float(np.abs(phi - phi_ref).max())
+    d_phi = (s1["sum_phi"] - s0["sum_phi"]) / abs(s0["sum_phi"])
+    d_rho = (s1["sum_rho"] - s0["sum_rho"]) / abs(s0["sum_rho"])
+    print(f"512^3 {prec}: max|dphi| {e_phi:.3e}, sum phi drift {d_phi:.2e}, "
+          f"sum rho drift {d_rho:.2e}; FP64 drift "
+          f"{(r1['sum_phi'] - r0['sum_phi']) / abs(r0['sum_phi']):.2e}", flush=True)
+    assert e_phi <= tol_phi
+    assert abs(d_phi) <= tol_sum and abs(d_rho) <= tol_sum
