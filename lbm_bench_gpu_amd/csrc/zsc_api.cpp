@@ -122,14 +122,16 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     c->rsize = precision == ZSC_F64 ? 8 : 4;
     // fused-kernel launch shape per precision, from 512^3 sweeps on MI355X:
     // FP32 gains from the register-held g prefetch (2 blocks per CU, 64
-    // planes); the 16-bit lattices, which are bound by memory-level
-    // parallelism, from 6 waves per SIMD (3 blocks per CU)
+    // planes); the 16-bit lattices, bound by memory-level parallelism, from
+    // two cells per lane (32-bit words: twice the bytes per load) where rows
+    // are a multiple of 128 cells, else from 6 waves per SIMD
+    const bool wide = nx % 128 == 0;
     if (precision == ZSC_F32)
         c->cfg = {8, 64, kFusedNtStore | kFusedPrefetch};
     else if (precision == ZSC_BF16)
-        c->cfg = {8, 32, kFusedNtStore};
+        c->cfg = wide ? FusedCfg{4, 24, kFusedNtStore | kFusedWide} : FusedCfg{8, 32, kFusedNtStore};
     else if (precision == ZSC_FP16)
-        c->cfg = {8, 16, kFusedNtStore};
+        c->cfg = wide ? FusedCfg{4, 24, kFusedNtStore | kFusedWide} : FusedCfg{8, 16, kFusedNtStore};
     else
         c->cfg = {4, 16, kFusedNtStore};
     hipError_t e = hipGetDevice(&c->device);
@@ -186,7 +188,7 @@ extern "C" int zsc_set_fused_config(zsc_ctx *c, int ty, int zchunk, int flags)
 {
     if (int r = need_ctx(c)) return r;
     if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || zchunk < 0 || flags < 0 ||
-        (flags & 0xf0) != 0 || (flags >> 8) > 128 || (flags & 3) == kFusedNtLoad)
+        (flags & 0xe0) != 0 || (flags >> 8) > 128 || (flags & 3) == kFusedNtLoad)
         return fail(ZSC_ERR_ARG, "zsc_set_fused_config: ty in {1,2,4,8}, zchunk >= 0, valid flags");
     c->cfg = {ty, zchunk, flags};
     return ZSC_OK;

@@ -756,6 +756,279 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fu
     }
 }
 
+// ---------------------------------------------------------------------------
+// The fused step for 16-bit storage with two cells per lane.
+//
+// A wave covers 128 consecutive cells of a row, lane l the cells x0 + 2l and
+// x0 + 2l + 1, so every population is read and written as aligned 32-bit
+// words holding a lane's two values: each wave load or store moves 256 bytes,
+// as in the FP32 kernel, instead of 128.  A pull across x is assembled in
+// registers: the neighbouring lane's word comes through a DPP wave shift, the
+// lane at the tile edge takes the neighbouring tile's word instead, and
+// v_alignbit joins the two halves.  Tiles are 128 x TY cells (TY = 4 or 8)
+// and need nx % 128 == 0.  The cell arithmetic is the one of k_fused.
+// ---------------------------------------------------------------------------
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// 32-bit word at (even) element e of a 16-bit population array.
+template <class T>
+__device__ inline uint32_t ld_w(const T *base, uint32_t e)
+{
+    return *(const uint32_t *)((const char *)base + (uint64_t)(e * 2u));
+}
+template <int FL, class T>
+__device__ inline void st_w(T *base, uint32_t e, uint32_t v)
+{
+    uint32_t *p = (uint32_t *)((char *)base + (uint64_t)(e * 2u));
+    if constexpr ((FL & kNtStore) != 0)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+// The word of a lane's two cells pulled from x - cx, given the word w of
+// the same two cells' own x positions (in the source row) and the word
+// `edge` of the neighbouring tile (x0 - 2 for cx = +1, x0 + 128 for cx = -1).
+template <int CX>
+__device__ inline uint32_t shift_word(uint32_t w, uint32_t edge)
+{
+    if constexpr (CX == 0) {
+        return w;
+    } else if constexpr (CX == 1) {
+        // lane l - 1's word (the edge word in lane 0): wave_shr:1
+        const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)w, 0x138, 0xF, 0xF, false);
+        return __builtin_amdgcn_alignbit(w, prev, 16);
+    } else {
+        // lane l + 1's word (the edge word in lane 63): wave_shl:1
+        const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)w, 0x130, 0xF, 0xF, false);
+        return __builtin_amdgcn_alignbit(next, w, 16);
+    }
+}
+
+template <class T>
+__device__ inline T half_of(uint32_t w, int j)
+{
+    T v;
+    v.v = (uint16_t)(j ? (w >> 16) : w);
+    return v;
+}
+
+// Start elements (x = 0) of the 3 x 3 rows (y + dy, z + dz), periodic.
+struct RowStarts {
+    uint32_t r[3][3];
+    __device__ RowStarts(int y, int z, int nx, int ny, int nz)
+    {
+        const int ys[3] = {wrap_dec(y, ny), y, wrap_inc(y, ny)};
+        const int zs[3] = {wrap_dec(z, nz), z, wrap_inc(z, nz)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                r[a][b] = (uint32_t)nx * ((uint32_t)ys[a] + (uint32_t)ny * (uint32_t)zs[b]);
+    }
+    // row of the source x - c: (y - cy, z - cz)
+    __device__ uint32_t src(int cy_, int cz_) const { return r[1 - cy_][1 - cz_]; }
+};
+
+// Pulled, decoded f of a lane's two cells (src/stream_f.cu:52-59 as a pull,
+// the operand order of pull_f32): f[j][i] for the cell x0 + 2l + j.
+template <class S, int FL>
+__device__ inline void pull_f_w(const Lattice<S> &B, int64_t n, const RowStarts &rs, uint32_t xo,
+                                uint32_t left, uint32_t right, const Coef<float> &c, float f[2][7])
+{
+    using SF = typename Traits<S>::SF;
+    const f32x2 f0 = *(const f32x2 *)((const char *)B.f0 + (uint64_t)((rs.r[1][1] + xo) * 4u));
+    f[0][0] = f0.x;
+    f[1][0] = f0.y;
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+        const SF *p = B.f + (int64_t)(i - 1) * n;
+        const uint32_t own = ld_w(p, rs.r[1][1] + xo);
+        const uint32_t row = rs.src(cy(i), cz(i));
+        uint32_t up;
+        if (cx(i) == 0)
+            up = ld_w(p, row + xo);
+        else if (cx(i) == 1)
+            up = shift_word<1>(ld_w(p, row + xo), ld_w(p, row + left));
+        else
+            up = shift_word<-1>(ld_w(p, row + xo), ld_w(p, row + right));
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            f[j][i] = c.eta * Codec<SF>::dec0(half_of<SF>(up, j)) +
+                      c.eta2 * Codec<SF>::dec0(half_of<SF>(own, j));
+    }
+}
+
+// 16-bit storage only: (16 waves per CU at 128 VGPRs: 2 blocks of 8 rows)
+template <class S, int TY, int FL>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)))
+    k_fused_w(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
+              Coef<float> c, Lattice<S> in, Lattice<S> out, float *__restrict__ phi_out)
+{
+    using R = float;
+    using SF = typename Traits<S>::SF;
+    using SG = typename Traits<S>::SG;
+    static_assert(sizeof(SG) == 2 && sizeof(SF) == 2 && TY >= 3, "16-bit storage, >= 3 rows");
+    // phi ring: [slot][row][cell parity][lane + 1]; x0 - 1 is [1][0], x0 + 128 is [0][65]
+    __shared__ R sp[4][TY + 2][2][66];
+    // each lane's pulled f of planes k and k+1 (thread-private slots)
+    __shared__ R fsh[2][7][TY][2][64];
+
+    const int b = blockIdx.x;
+    int l = b;
+    if (remap && (nblk & 7) == 0) l = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int bx = l % nbx;
+    const int by = (l / nbx) % nby;
+    const int bz = l / (nbx * nby);
+    const int x0 = bx * 128, y0 = by * TY, z0 = bz * zchunk;
+    const int z1 = min(nz, z0 + zchunk);
+    const int th = min(TY, ny - y0);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const bool own = ty < th;
+    const int y = y0 + min(ty, th - 1);
+    const uint32_t xo = (uint32_t)(x0 + 2 * tx);
+    const uint32_t left = (uint32_t)(x0 == 0 ? nx - 2 : x0 - 2);
+    const uint32_t right = (uint32_t)(x0 + 128 == nx ? 0 : x0 + 128);
+
+    auto plane_phi = [&](int zz) {
+        const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        const int slot = (zz + 4) & 3;
+        if (own) {
+            float f[2][7];
+            pull_f_w<S, FL>(in, n, RowStarts(y, zc, nx, ny, nz), xo, left, right, c, f);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                sp[slot][ty + 1][j][tx + 1] = sum_f(f[j]);
+#pragma unroll
+                for (int i = 0; i < 7; ++i) fsh[(zz + 2) & 1][i][ty][j][tx] = f[j][i];
+            }
+        }
+        // ring rows y0 - 1 (wave 0) and y0 + th (wave 1): whole rows
+        if (ty < 2) {
+            const int r = ty == 0 ? 0 : th + 1;
+            int hy = ty == 0 ? y0 - 1 : y0 + th;
+            hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+            float f[2][7];
+            pull_f_w<S, FL>(in, n, RowStarts(hy, zc, nx, ny, nz), xo, left, right, c, f);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) sp[slot][r][j][tx + 1] = sum_f(f[j]);
+        }
+        // ring columns x0 - 1 and x0 + 128, rows y0 - 1 .. y0 + th (wave 2)
+        if (ty == 2 && tx < 2 * (th + 2)) {
+            const int side = tx / (th + 2), r = tx % (th + 2);
+            int hx = side == 0 ? x0 - 1 : x0 + 128;
+            hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
+            int hy = y0 - 1 + r;
+            hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+            R f[7];
+            pull_f32<S, R, FL>(in, n, Nbr32(hx, hy, zc, nx, ny, nz), c, f);
+            if (side == 0)
+                sp[slot][r][1][0] = sum_f(f);
+            else
+                sp[slot][r][0][65] = sum_f(f);
+        }
+    };
+
+    __builtin_amdgcn_s_setprio(2);
+    plane_phi(z0 - 1);
+    plane_phi(z0);
+    for (int k = z0; k < z1; ++k) {
+#if ZSC_OPAQUE_STRIDE
+        {
+            uint32_t lo = (uint32_t)n;
+            asm volatile("" : "+s"(lo));
+            n = (int64_t)lo;
+        }
+#endif
+        __builtin_amdgcn_s_setprio(2);
+        // plane k's pulled g as stored words (shifted at the collision)
+        uint32_t gw[19], ge[19];
+        if (own) {
+            const RowStarts rs(y, k, nx, ny, nz);
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const SG *p = in.g + (int64_t)i * n;
+                const uint32_t row = rs.src(cy(i), cz(i));
+                gw[i] = ld_w(p, row + xo);
+                if (cx(i) == 1) ge[i] = ld_w(p, row + left);
+                if (cx(i) == -1) ge[i] = ld_w(p, row + right);
+            }
+        }
+        plane_phi(k + 1);
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                if (cx(i) == 1) gw[i] = shift_word<1>(gw[i], ge[i]);
+                if (cx(i) == -1) gw[i] = shift_word<-1>(gw[i], ge[i]);
+            }
+            const int sm = (k + 3) & 3, s0 = k & 3, spl = (k + 1) & 3;
+            const int r = ty + 1;
+            const uint32_t orow = (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)k) + xo;
+            uint32_t lo16[25];  // cell 0's 16-bit outputs: f1..6, g0..18
+            float f0lo = 0.f;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                // x offsets of the stencil in [parity][lane + 1] form
+                const int jm = j ^ 1, qm = tx + j, jp = j ^ 1, qp = tx + 1 + j, q = tx + 1;
+                R p[19];
+                p[0] = sp[s0][r][j][q];
+                p[1] = sp[s0][r][jp][qp];
+                p[2] = sp[s0][r][jm][qm];
+                p[3] = sp[s0][r + 1][j][q];
+                p[4] = sp[s0][r - 1][j][q];
+                p[5] = sp[spl][r][j][q];
+                p[6] = sp[sm][r][j][q];
+                p[7] = sp[s0][r + 1][jp][qp];
+                p[8] = sp[s0][r - 1][jm][qm];
+                p[9] = sp[s0][r - 1][jp][qp];
+                p[10] = sp[s0][r + 1][jm][qm];
+                p[11] = sp[spl][r][jp][qp];
+                p[12] = sp[sm][r][jm][qm];
+                p[13] = sp[sm][r][jp][qp];
+                p[14] = sp[spl][r][jm][qm];
+                p[15] = sp[spl][r + 1][j][q];
+                p[16] = sp[sm][r - 1][j][q];
+                p[17] = sp[sm][r + 1][j][q];
+                p[18] = sp[spl][r - 1][j][q];
+                R fcur[7], ga[19], fb[7], gb[19];
+#pragma unroll
+                for (int i = 0; i < 7; ++i) fcur[i] = fsh[(k + 2) & 1][i][ty][j][tx];
+#pragma unroll
+                for (int i = 0; i < 19; ++i) ga[i] = Codec<SG>::dec0(half_of<SG>(gw[i], j));
+                collide_cell_dev(c, p, fcur, ga, fb, gb);
+                if (phi_out) phi_out[orow + j] = p[0];
+                if (j == 0) {
+                    f0lo = fb[0];
+#pragma unroll
+                    for (int i = 1; i < 7; ++i) lo16[i - 1] = Codec<SF>::enc0(fb[i]).v;
+#pragma unroll
+                    for (int i = 0; i < 19; ++i) lo16[6 + i] = Codec<SG>::enc0(gb[i]).v;
+                } else {
+                    f32x2 f0v;
+                    f0v.x = f0lo;
+                    f0v.y = fb[0];
+                    f32x2 *f0p = (f32x2 *)((char *)out.f0 + (uint64_t)(orow * 4u));
+                    if constexpr ((FL & kNtStore) != 0)
+                        __builtin_nontemporal_store(f0v, f0p);
+                    else
+                        *f0p = f0v;
+#pragma unroll
+                    for (int i = 1; i < 7; ++i)
+                        st_w<FL>(out.f + (int64_t)(i - 1) * n, orow,
+                                 lo16[i - 1] | ((uint32_t)Codec<SF>::enc0(fb[i]).v << 16));
+#pragma unroll
+                    for (int i = 0; i < 19; ++i)
+                        st_w<FL>(out.g + (int64_t)i * n, orow,
+                                 lo16[6 + i] | ((uint32_t)Codec<SG>::enc0(gb[i]).v << 16));
+                }
+            }
+        }
+    }
+}
+
 // FP64 reductions for zsc_stats (src/stats.c:39-55, src/logsave.c:63-143).
 template <class R>
 __global__ void __launch_bounds__(256)
@@ -933,6 +1206,26 @@ static hipError_t launch_fused_v(const Geo &g, const Coef<typename Traits<S>::R>
     return hipGetLastError();
 }
 
+template <class S, int TY, int FL>
+static hipError_t launch_fused_w(const Geo &g, const Coef<typename Traits<S>::R> &c,
+                                 hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout,
+                                 void *phi_out)
+{
+    int zchunk = cfg.zchunk > 0 ? cfg.zchunk : g.nz;
+    const int nbx = g.nx / 128, nby = (g.ny + TY - 1) / TY;
+    while ((int64_t)nbx * nby * ((g.nz + zchunk - 1) / zchunk) < 2048 && zchunk > 4) zchunk /= 2;
+    const int nbz = (g.nz + zchunk - 1) / zchunk;
+    const int64_t nblk = (int64_t)nbx * nby * nbz;
+    if (nblk > 0x7fffffff || g.nx % 128 != 0) return hipErrorInvalidValue;
+    if ((uint64_t)g.n * sizeof(typename Traits<S>::S0) > ((uint64_t)1 << 32)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fused_w<S, TY, FL>), dim3((unsigned)nblk), dim3(64, TY), 0, s, g.nx, g.ny,
+                       g.nz, g.n, zchunk, nbx, nby, (int)nblk,
+                       (cfg.flags & kFusedIdentityMap) ? 0 : 1, c,
+                       LatLayout<S>::view(const_cast<void *>(Bin), g.n),
+                       LatLayout<S>::view(Bout, g.n), (float *)phi_out);
+    return hipGetLastError();
+}
+
 template <class S, int TY>
 static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R> &c,
                                   hipStream_t s, const FusedCfg &cfg, const void *Bin, void *Bout,
@@ -943,6 +1236,13 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
         if (cfg.flags & kNtStore) return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);
         return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
     } else {
+        if constexpr (sizeof(typename Traits<S>::SG) == 2 && (TY == 4 || TY == 8)) {
+            // 16-bit storage, two cells per lane (rows of 128 cells)
+            if ((cfg.flags & kFusedWide) && g.nx % 128 == 0) {
+                if (cfg.flags & kNtStore) return launch_fused_w<S, TY, 1>(g, c, s, cfg, Bin, Bout, phi);
+                return launch_fused_w<S, TY, 0>(g, c, s, cfg, Bin, Bout, phi);
+            }
+        }
         const int fl = cfg.flags & (kNtStore | kNtLoad);
         const bool pf = (cfg.flags & kFusedPrefetch) != 0;
         switch (fl + (pf ? 4 : 0)) {
