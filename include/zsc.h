@@ -243,8 +243,8 @@ int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
  * flags: 1 non-temporal output stores, 2 (with 1) non-temporal loads, 4
  * prefetch the next plane's g populations (software pipelining: g_0..g_5 for
  * FP32 storage, all 19 for 16-bit storage), 8 no
- * XCD-contiguous tile order, 16 (16-bit storage, ty 4 or 8, nx a multiple
- * of 128; ignored otherwise) two cells per lane, and bits
+ * XCD-contiguous tile order, 16 (FP32-compute precisions, ty 4 or 8, nx a
+ * multiple of 128; ignored otherwise) two cells per lane, and bits
  * 8..15 a number of KiB (<= 128) of unused LDS per block that limits how
  * many blocks share a CU.  Bits 1-2 only apply to the FP32-compute
  * precisions.  No reference counterpart; a

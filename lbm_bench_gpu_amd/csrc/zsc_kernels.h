@@ -814,6 +814,101 @@ __device__ inline T half_of(uint32_t w, int j)
     return v;
 }
 
+// A lane's two consecutive elements of a population array: one 32-bit word
+// for 16-bit storage, a float pair for FP32 storage.
+template <class T>
+struct Pair {
+    using type = uint32_t;
+};
+template <>
+struct Pair<float> {
+    using type = f32x2;
+};
+
+template <class T>
+__device__ inline typename Pair<T>::type ld_pair(const T *base, uint32_t e)
+{
+    if constexpr (sizeof(T) == 2)
+        return ld_w(base, e);
+    else
+        return *(const f32x2 *)((const char *)base + (uint64_t)(e * 4u));
+}
+template <int FL, class T>
+__device__ inline void st_pair(T *base, uint32_t e, typename Pair<T>::type v)
+{
+    if constexpr (sizeof(T) == 2) {
+        st_w<FL>(base, e, v);
+    } else {
+        f32x2 *p = (f32x2 *)((char *)base + (uint64_t)(e * 4u));
+        if constexpr ((FL & kNtStore) != 0)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+}
+
+// The pair pulled from x - cx (see shift_word); for FP32 only one element
+// crosses lanes.  (The elements are copied to scalars before the bit casts:
+// a bit cast of a vector element expression reads the vector's first
+// element.)
+template <int CX>
+__device__ inline f32x2 shift_word(f32x2 w, f32x2 edge)
+{
+    const float wx = w.x, wy = w.y;
+    if constexpr (CX == 0) {
+        return w;
+    } else if constexpr (CX == 1) {
+        const float ey = edge.y;
+        const float prev = __builtin_bit_cast(
+            float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, ey), __builtin_bit_cast(int, wy),
+                                               0x138, 0xF, 0xF, false));
+        f32x2 r;
+        r.x = prev;
+        r.y = wx;
+        return r;
+    } else {
+        const float ex = edge.x;
+        const float next = __builtin_bit_cast(
+            float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, ex), __builtin_bit_cast(int, wx),
+                                               0x130, 0xF, 0xF, false));
+        f32x2 r;
+        r.x = wy;
+        r.y = next;
+        return r;
+    }
+}
+
+// Element j of a pair, as stored (16-bit: still encoded).
+template <class T>
+__device__ inline auto elem(typename Pair<T>::type w, int j)
+{
+    if constexpr (sizeof(T) == 2)
+        return half_of<T>(w, j);
+    else
+        return j ? w.y : w.x;
+}
+// A cell's output encoded for storage, and two cells' outputs joined.
+template <class T>
+__device__ inline auto enc_out(float v)
+{
+    if constexpr (sizeof(T) == 2)
+        return (uint32_t)Codec<T>::enc0(v).v;
+    else
+        return v;
+}
+template <class T, class E>
+__device__ inline typename Pair<T>::type join(E a, E b)
+{
+    if constexpr (sizeof(T) == 2) {
+        return a | (b << 16);
+    } else {
+        f32x2 r;
+        r.x = a;
+        r.y = b;
+        return r;
+    }
+}
+
 // Start elements (x = 0) of the 3 x 3 rows (y + dy, z + dz), periodic.
 struct RowStarts {
     uint32_t r[3][3];
@@ -844,32 +939,33 @@ __device__ inline void pull_f_w(const Lattice<S> &B, int64_t n, const RowStarts 
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
         const SF *p = B.f + (int64_t)(i - 1) * n;
-        const uint32_t own = ld_w(p, rs.r[1][1] + xo);
+        const auto own = ld_pair(p, rs.r[1][1] + xo);
         const uint32_t row = rs.src(cy(i), cz(i));
-        uint32_t up;
+        typename Pair<SF>::type up;
         if (cx(i) == 0)
-            up = ld_w(p, row + xo);
+            up = ld_pair(p, row + xo);
         else if (cx(i) == 1)
-            up = shift_word<1>(ld_w(p, row + xo), ld_w(p, row + left));
+            up = shift_word<1>(ld_pair(p, row + xo), ld_pair(p, row + left));
         else
-            up = shift_word<-1>(ld_w(p, row + xo), ld_w(p, row + right));
+            up = shift_word<-1>(ld_pair(p, row + xo), ld_pair(p, row + right));
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-            f[j][i] = c.eta * Codec<SF>::dec0(half_of<SF>(up, j)) +
-                      c.eta2 * Codec<SF>::dec0(half_of<SF>(own, j));
+            f[j][i] = c.eta * Codec<SF>::dec0(elem<SF>(up, j)) +
+                      c.eta2 * Codec<SF>::dec0(elem<SF>(own, j));
     }
 }
 
-// 16-bit storage only: (16 waves per CU at 128 VGPRs: 2 blocks of 8 rows)
+// LDS holds 3 blocks of 4 rows (12 waves) or 2 of 8 rows (16 waves) per CU;
+// the register budget matches.
 template <class S, int TY, int FL>
-__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY == 4 ? 3 : 4)))
     k_fused_w(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk, int remap,
               Coef<float> c, Lattice<S> in, Lattice<S> out, float *__restrict__ phi_out)
 {
     using R = float;
     using SF = typename Traits<S>::SF;
     using SG = typename Traits<S>::SG;
-    static_assert(sizeof(SG) == 2 && sizeof(SF) == 2 && TY >= 3, "16-bit storage, >= 3 rows");
+    static_assert(sizeof(SG) == sizeof(SF) && TY >= 3, "FP32 or 16-bit storage, >= 3 rows");
     // phi ring: [slot][row][cell parity][lane + 1]; x0 - 1 is [1][0], x0 + 128 is [0][65]
     __shared__ R sp[4][TY + 2][2][66];
     // each lane's pulled f of planes k and k+1 (thread-private slots)
@@ -943,16 +1039,16 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
 #endif
         __builtin_amdgcn_s_setprio(2);
         // plane k's pulled g as stored words (shifted at the collision)
-        uint32_t gw[19], ge[19];
+        typename Pair<SG>::type gw[19], ge[19];
         if (own) {
             const RowStarts rs(y, k, nx, ny, nz);
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
                 const SG *p = in.g + (int64_t)i * n;
                 const uint32_t row = rs.src(cy(i), cz(i));
-                gw[i] = ld_w(p, row + xo);
-                if (cx(i) == 1) ge[i] = ld_w(p, row + left);
-                if (cx(i) == -1) ge[i] = ld_w(p, row + right);
+                gw[i] = ld_pair(p, row + xo);
+                if (cx(i) == 1) ge[i] = ld_pair(p, row + left);
+                if (cx(i) == -1) ge[i] = ld_pair(p, row + right);
             }
         }
         plane_phi(k + 1);
@@ -967,7 +1063,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
             const int sm = (k + 3) & 3, s0 = k & 3, spl = (k + 1) & 3;
             const int r = ty + 1;
             const uint32_t orow = (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)k) + xo;
-            uint32_t lo16[25];  // cell 0's 16-bit outputs: f1..6, g0..18
+            using OE = decltype(enc_out<SG>(0.f));
+            OE lo16[25];  // cell 0's outputs as stored: f1..6, g0..18
             float f0lo = 0.f;
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
@@ -997,15 +1094,23 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
                 for (int i = 0; i < 7; ++i) fcur[i] = fsh[(k + 2) & 1][i][ty][j][tx];
 #pragma unroll
-                for (int i = 0; i < 19; ++i) ga[i] = Codec<SG>::dec0(half_of<SG>(gw[i], j));
-                collide_cell_dev(c, p, fcur, ga, fb, gb);
+                for (int i = 0; i < 19; ++i) {
+                    if constexpr (Codec<SG>::deviation)
+                        ga[i] = Codec<SG>::dec0(elem<SG>(gw[i], j));
+                    else
+                        ga[i] = Codec<SG>::dec(elem<SG>(gw[i], j), g_rest(c, i));
+                }
+                if constexpr (Codec<SG>::deviation)
+                    collide_cell_dev(c, p, fcur, ga, fb, gb);
+                else
+                    collide_cell(c, p, fcur, ga, fb, gb);
                 if (phi_out) phi_out[orow + j] = p[0];
                 if (j == 0) {
                     f0lo = fb[0];
 #pragma unroll
-                    for (int i = 1; i < 7; ++i) lo16[i - 1] = Codec<SF>::enc0(fb[i]).v;
+                    for (int i = 1; i < 7; ++i) lo16[i - 1] = enc_out<SF>(fb[i]);
 #pragma unroll
-                    for (int i = 0; i < 19; ++i) lo16[6 + i] = Codec<SG>::enc0(gb[i]).v;
+                    for (int i = 0; i < 19; ++i) lo16[6 + i] = enc_out<SG>(gb[i]);
                 } else {
                     f32x2 f0v;
                     f0v.x = f0lo;
@@ -1017,12 +1122,12 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
                         *f0p = f0v;
 #pragma unroll
                     for (int i = 1; i < 7; ++i)
-                        st_w<FL>(out.f + (int64_t)(i - 1) * n, orow,
-                                 lo16[i - 1] | ((uint32_t)Codec<SF>::enc0(fb[i]).v << 16));
+                        st_pair<FL>(out.f + (int64_t)(i - 1) * n, orow,
+                                    join<SF>(lo16[i - 1], enc_out<SF>(fb[i])));
 #pragma unroll
                     for (int i = 0; i < 19; ++i)
-                        st_w<FL>(out.g + (int64_t)i * n, orow,
-                                 lo16[6 + i] | ((uint32_t)Codec<SG>::enc0(gb[i]).v << 16));
+                        st_pair<FL>(out.g + (int64_t)i * n, orow,
+                                    join<SG>(lo16[6 + i], enc_out<SG>(gb[i])));
                 }
             }
         }
@@ -1236,8 +1341,8 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
         if (cfg.flags & kNtStore) return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);
         return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
     } else {
-        if constexpr (sizeof(typename Traits<S>::SG) == 2 && (TY == 4 || TY == 8)) {
-            // 16-bit storage, two cells per lane (rows of 128 cells)
+        if constexpr (TY == 4 || TY == 8) {
+            // two cells per lane (rows of 128 cells)
             if ((cfg.flags & kFusedWide) && g.nx % 128 == 0) {
                 if (cfg.flags & kNtStore) return launch_fused_w<S, TY, 1>(g, c, s, cfg, Bin, Bout, phi);
                 return launch_fused_w<S, TY, 0>(g, c, s, cfg, Bin, Bout, phi);

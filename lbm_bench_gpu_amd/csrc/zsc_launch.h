@@ -42,7 +42,7 @@ enum : int {
     kFusedNtLoad = 2,       // non-temporal loads (only together with kFusedNtStore)
     kFusedPrefetch = 4,     // load (part of) the next plane's g during this plane
     kFusedIdentityMap = 8,  // no XCD-contiguous tile remapping
-    kFusedWide = 16,        // 16-bit storage: two cells per lane (nx % 128 == 0, ty 4 or 8)
+    kFusedWide = 16,        // two cells per lane (nx % 128 == 0, ty 4 or 8; not FP64)
 };
 struct FusedCfg {
     int ty;      // tile rows (waves per block); tile is 64 x ty cells

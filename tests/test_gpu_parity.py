@@ -263,19 +263,20 @@ def test_launch_shapes_agree(need_gpu, prec, tol):
             assert ef <= tol and eg <= tol, (ty, zc, fl, ef, eg)
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-6), ("bf16", 1e-4), ("fp16", 1e-4)])
 @pytest.mark.parametrize("dims,ty,zc", [((128, 20, 12), 8, 5), ((256, 9, 7), 8, 0),
                                         ((128, 6, 9), 4, 4), ((384, 13, 5), 4, 2),
                                         ((128, 1, 1), 4, 0), ((128, 2, 3), 4, 24)])
-def test_two_cells_per_lane_matches_per_cell_kernel(need_gpu, prec, dims, ty, zc):
-    """The 16-bit kernel with two cells per lane (flag 16: aligned 32-bit
-    words, x pulls assembled by DPP wave shifts and the neighbouring tile's
-    edge word) against the one-cell-per-lane kernel: partial y tiles, a last
+def test_two_cells_per_lane_matches_per_cell_kernel(need_gpu, prec, tol, dims, ty, zc):
+    """The kernel with two cells per lane (flag 16: aligned 32-bit words or
+    float pairs, x pulls assembled by DPP wave shifts and the neighbouring
+    tile's edge word) against the one-cell-per-lane kernel: partial y tiles, a last
     tile of one row, one- and two-row boxes and a one-plane box (the periodic
     wrap folds the ring onto the tile), whole-box and short z chunks,
     relaxation then gravity.
     Both kernels run the same cell arithmetic; FMA contraction may differ per
-    kernel, so the bound is the launch-shape one (1e-4 of max |f|)."""
+    kernel, so the bound is the launch-shape one (FP32 1e-6, 16-bit storage
+    1e-4 of max |f|)."""
     p, phi0 = small_case(dims, nb=3, seed=23)
     out = []
     for flags in (1, 1 | 16):
@@ -290,7 +291,7 @@ def test_two_cells_per_lane_matches_per_cell_kernel(need_gpu, prec, dims, ty, zc
             out.append(ctx.state_get())
     ef, eg = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
     print(prec, dims, ty, zc, "wide vs per-cell", ef, eg, flush=True)
-    assert ef <= 1e-4 and eg <= 1e-4
+    assert ef <= tol and eg <= tol
 
 
 @pytest.mark.parametrize("dims,cfg", [
