@@ -11,6 +11,9 @@ depend on size.
   storage the bulk cells, all alike, round their over-relaxed g update the
   same way, so the mass error adds up to 1.6e-6 before the rounded map
   settles; FP64 conserves to 1e-15).
+* 1024^3, 512 bubbles (seed 3), bf16 storage: a few steps of the
+  two-cells-per-lane kernel at its 32-bit offset limit conserve total phi and
+  mass.
 """
 import numpy as np
 import pytest
@@ -82,3 +85,25 @@ def test_512_box_reduced_precision_vs_fp64(ref_512, prec, tol_phi, tol_sum):
           f"{(r1['sum_phi'] - r0['sum_phi']) / abs(r0['sum_phi']):.2e}", flush=True)
     assert e_phi <= tol_phi
     assert abs(d_phi) <= tol_sum and abs(d_rho) <= tol_sum
+
+
+def test_1024_box_bf16_two_cells_per_lane(need_gpu):
+    """1024^3 bf16 (the two-cells-per-lane kernel at the 32-bit offset limit
+    of its FP32 f_0 pairs, 2^32 - 8 bytes; 137 GB of lattices and work
+    arrays): five steps conserve total phi and mass to round-off and keep the
+    phase volume."""
+    w = configs.workload(3)
+    p = w.params()
+    with zsc.Context(*w.dims, precision="bf16") as ctx:
+        ctx.set_params(p)
+        ctx.state_init_bubbles(w.bubbles)
+        s0 = ctx.stats()
+        ctx.step(5)
+        s1 = ctx.stats()
+    d_phi = (s1["sum_phi"] - s0["sum_phi"]) / abs(s0["sum_phi"])
+    d_rho = (s1["sum_rho"] - s0["sum_rho"]) / abs(s0["sum_rho"])
+    v = s1["bubble_volume_full"] / s0["bubble_volume_full"]
+    print(f"1024^3 bf16: sum phi drift {d_phi:.2e}, sum rho drift {d_rho:.2e}, volume ratio {v:.6f}",
+          flush=True)
+    assert abs(d_phi) <= 1e-6 and abs(d_rho) <= 1e-6
+    assert abs(v - 1) < 1e-3
