@@ -124,7 +124,7 @@ inline dim3 grid_for(int64_t n, int block = 256)
 // ===========================================================================
 
 // Memory-access flags of the fused kernel: streaming (non-temporal) hints.
-enum : int { kNtStore = 1, kNtLoad = 2 };
+enum : int { kNtStore = 1, kNtLoad = 2, kZStash = 4 };
 
 template <int FL, class S>
 __device__ inline S ld_raw(const S *p)
@@ -312,6 +312,40 @@ __device__ inline void pull_f32(const Lattice<S> &B, int64_t n, const NB &nb, co
         const R up = Codec<SF>::dec0(
             ldx<FL>(p, nb.off(-cx(i), -cy(i), -cz(i)), NB::xstep(-cx(i))));
         f[i] = c.eta * up + c.eta2 * ld32z<FL>(p, own);
+    }
+}
+
+// pull_f32 for a cell whose previous plane was pulled by the same thread:
+// f_5's upwind value (plane z-1) is that plane's own f_5, and f_6's own value
+// is that plane's upwind f_6 (plane z); both come from the thread's stash
+// (s5, s6) when `use`, and the stash is refilled for the next plane.
+template <class S, class R, int FL, class NB>
+__device__ inline void pull_f32_zs(const Lattice<S> &B, int64_t n, const NB &nb, const Coef<R> &c,
+                                   R *f, R &s5, R &s6, bool use)
+{
+    using SF = typename Traits<S>::SF;
+    const uint32_t own = nb.at(0, 0, 0);
+    f[0] = ld32z<FL>(B.f0, own);
+#pragma unroll
+    for (int i = 1; i < 5; ++i) {
+        const auto *p = B.f + (int64_t)(i - 1) * n;
+        const R up = Codec<SF>::dec0(
+            ldx<FL>(p, nb.off(-cx(i), -cy(i), -cz(i)), NB::xstep(-cx(i))));
+        f[i] = c.eta * up + c.eta2 * ld32z<FL>(p, own);
+    }
+    {
+        const auto *p = B.f + (int64_t)4 * n;
+        const R own5 = ld32z<FL>(p, own);
+        const R up5 = use ? s5 : Codec<SF>::dec0(ldx<FL>(p, nb.off(0, 0, -1), 0));
+        f[5] = c.eta * up5 + c.eta2 * own5;
+        s5 = own5;
+    }
+    {
+        const auto *p = B.f + (int64_t)5 * n;
+        const R up6 = Codec<SF>::dec0(ldx<FL>(p, nb.off(0, 0, 1), 0));
+        const R own6 = use ? s6 : ld32z<FL>(p, own);
+        f[6] = c.eta * up6 + c.eta2 * own6;
+        s6 = up6;
     }
 }
 
@@ -571,6 +605,11 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fu
     // held across the z loop; this is what lets the kernel fit in 80.
     constexpr bool kStash = sizeof(R) == 4;
     __shared__ R fsh[kStash ? 2 : 1][kStash ? 7 : 1][kStash ? TY : 1][64];
+    // f_5 / f_6 values shared by consecutive planes (thread-private slots)
+    constexpr bool kZS = (FL & kZStash) != 0;
+    constexpr int kRing = 2 * 64 + 2 * (TY + 2);
+    __shared__ R zs[kZS ? 2 : 1][kZS ? TY : 1][64];
+    __shared__ R zsr[kZS ? 2 : 1][kZS ? kRing : 1];
 
     // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tiles so that neighbouring rows share its L2.
@@ -594,14 +633,23 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fu
     const int tid = ty * 64 + tx;
 
     // phi of plane zz (tile + ring) into LDS slot; own pulled f into fown.
-    auto plane_phi = [&](int zz, R *fown) {
+    auto plane_phi = [&](int zz, R *fown, bool use) {
         const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
         const int slot = (zz + 4) & 3;
         if (own) {
-            if (xint)
-                pull_f32<S, R, FL>(in, n, NbrRow(x, y, zc, nx, ny, nz), c, fown);
-            else
-                pull_f32<S, R, FL>(in, n, Nbr32(x, y, zc, nx, ny, nz), c, fown);
+            if constexpr (kZS) {
+                if (xint)
+                    pull_f32_zs<S, R, FL>(in, n, NbrRow(x, y, zc, nx, ny, nz), c, fown, zs[0][ty][tx],
+                                          zs[1][ty][tx], use);
+                else
+                    pull_f32_zs<S, R, FL>(in, n, Nbr32(x, y, zc, nx, ny, nz), c, fown, zs[0][ty][tx],
+                                          zs[1][ty][tx], use);
+            } else {
+                if (xint)
+                    pull_f32<S, R, FL>(in, n, NbrRow(x, y, zc, nx, ny, nz), c, fown);
+                else
+                    pull_f32<S, R, FL>(in, n, Nbr32(x, y, zc, nx, ny, nz), c, fown);
+            }
             sp[slot][ty + 1][tx + 1] = sum_f(fown);
             if constexpr (kStash) {
 #pragma unroll
@@ -634,10 +682,19 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fu
             hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
             hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
             R f[7];
-            if (xint_ring)
-                pull_f32<S, R, FL>(in, n, NbrRow(hx, hy, zc, nx, ny, nz), c, f);
-            else
-                pull_f32<S, R, FL>(in, n, Nbr32(hx, hy, zc, nx, ny, nz), c, f);
+            if constexpr (kZS) {
+                if (xint_ring)
+                    pull_f32_zs<S, R, FL>(in, n, NbrRow(hx, hy, zc, nx, ny, nz), c, f, zsr[0][h],
+                                          zsr[1][h], use);
+                else
+                    pull_f32_zs<S, R, FL>(in, n, Nbr32(hx, hy, zc, nx, ny, nz), c, f, zsr[0][h],
+                                          zsr[1][h], use);
+            } else {
+                if (xint_ring)
+                    pull_f32<S, R, FL>(in, n, NbrRow(hx, hy, zc, nx, ny, nz), c, f);
+                else
+                    pull_f32<S, R, FL>(in, n, Nbr32(hx, hy, zc, nx, ny, nz), c, f);
+            }
             sp[slot][r][col] = sum_f(f);
         }
     };
@@ -662,8 +719,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fu
     // by the CU's arbiter over waves in their collision arithmetic
     // (default), so loads leave early and more bytes are in flight.
     __builtin_amdgcn_s_setprio(2);
-    plane_phi(z0 - 1, fcur);  // phi only; its own f is not collided
-    plane_phi(z0, fcur);
+    plane_phi(z0 - 1, fcur, false);  // phi only; its own f is not collided
+    plane_phi(z0, fcur, true);
     for (int k = z0; k < z1; ++k) {
 #if ZSC_OPAQUE_STRIDE
         // Keep the 52 per-population base addresses out of loop-invariant
@@ -708,9 +765,9 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(fu
         }
         if constexpr (kStash) {
             R ftmp[7];
-            plane_phi(k + 1, ftmp);
+            plane_phi(k + 1, ftmp, true);
         } else {
-            plane_phi(k + 1, fnext);
+            plane_phi(k + 1, fnext, true);
         }
         __syncthreads();
         __builtin_amdgcn_s_setprio(0);
@@ -1350,6 +1407,10 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
         }
         const int fl = cfg.flags & (kNtStore | kNtLoad);
         const bool pf = (cfg.flags & kFusedPrefetch) != 0;
+        if ((cfg.flags & kFusedZStash) && fl == kNtStore) {
+            if (pf) return launch_fused_v<S, TY, kNtStore | kZStash, true>(g, c, s, cfg, Bin, Bout, phi);
+            return launch_fused_v<S, TY, kNtStore | kZStash, false>(g, c, s, cfg, Bin, Bout, phi);
+        }
         switch (fl + (pf ? 4 : 0)) {
         case 0: return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
         case 1: return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);

@@ -239,7 +239,8 @@ def test_bubble_init_on_device_matches_host(need_gpu):
 
 @pytest.mark.parametrize("prec,tol", [("f64", 0.0), ("f32", 1e-6), ("bf16", 1e-4), ("fp16", 1e-4)])
 def test_launch_shapes_agree(need_gpu, prec, tol):
-    """Tile rows, z chunks, streaming hints, prefetch and LDS padding change
+    """Tile rows, z chunks, streaming hints, prefetch, LDS padding and the
+    f_5/f_6 plane stash (flag 32) change
     only the schedule.  FP64 (no FMA contraction) gives the same bits for
     every shape; the FP32-compute kernels are contracted into FMAs per
     template instance, so shapes may differ in the last FP32 bit, which a
@@ -247,7 +248,7 @@ def test_launch_shapes_agree(need_gpu, prec, tol):
     dims = (70, 18, 12)
     p, phi0 = small_case(dims, seed=13)
     shapes = [(4, 16, 0), (8, 16, 1), (2, 3, 5), (1, 0, 4), (4, 5, 1 | (16 << 8)), (8, 2, 8),
-              (8, 64, 5)]
+              (8, 64, 5), (8, 64, 5 | 32), (4, 3, 1 | 32), (1, 0, 1 | 32)]
     ref = None
     for ty, zc, fl in shapes:
         with zsc.Context(*dims, precision=prec) as ctx:
