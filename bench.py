@@ -143,11 +143,12 @@ def accuracy(w, params, steps, precs, ty, zchunk, flags):
 
 
 def adjoint_timing(steps: int, fwd_mlups_fp32: float):
-    """MLUPS of the checkpointed adjoint on the 128^3, 8-bubble workload
-    (FP32, checkpoints every 20 steps): cells * steps / time of the backward
-    pass including the recomputation of every segment, and of the adjoint
-    steps alone (HIP events around the recomputation and the backward steps
-    of each segment)."""
+    """MLUPS of the adjoint on the 128^3, 8-bubble workload (FP32): cells *
+    steps / time of the backward pass, once with the tape of every step in
+    HBM (nothing to recompute; 90 GB at 400 steps) and once with checkpoints
+    every 20 steps (the recomputation of every segment included), and of
+    the adjoint steps alone (HIP events around each segment's recomputation
+    and its backward steps).  Both modes give the same gradients."""
     import torch
     from lbm_bench_gpu_amd import adjoint, configs, zsc
     w = configs.workload(4)
@@ -155,15 +156,6 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     phi0 = torch.from_numpy(zsc.init_phi_host(w.dims, w.bubbles, p.phi_star, p.width))
     theta = adjoint.theta_of(p)
     tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
-    run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20, timing=True)
-    tgt = run.forward(tt, phi0, steps, keep=False)[:7].sum(0).clone()
-    t0 = time.perf_counter()
-    a = run.forward(theta, phi0, steps)
-    t1 = time.perf_counter()
-    L, lam = run.loss(a, tgt)
-    t2 = time.perf_counter()
-    grads = run.backward(lam)
-    t3 = time.perf_counter()
     # forward MLUPS of the fused FP32 step on the same box
     ctx = zsc.Context(*w.dims, precision="f32")
     ctx.set_params(p)
@@ -172,18 +164,40 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     ms = ctx.step_timed(steps)
     ctx.close()
     fused = w.cells * steps / (ms / 1e3) / 1e6
-    adj = w.cells * steps / (t3 - t2) / 1e6
-    tm = run.last_timing
-    adj_steps = w.cells * steps / (tm["adjoint_steps_ms"] / 1e3) / 1e6
-    return {"workload": w.name, "steps": steps, "checkpoint_every": 20, "precision": "f32",
-            "loss": L, "grad": dict(zip(adjoint.THETA, grads)),
-            "forward_with_checkpoints_mlups": round(w.cells * steps / (t1 - t0) / 1e6, 1),
-            "adjoint_mlups": round(adj, 1), "fused_forward_mlups": round(fused, 1),
-            "adjoint_vs_forward": round(adj / fused, 3),
-            "adjoint_step_mlups": round(adj_steps, 1),
-            "adjoint_step_vs_forward": round(adj_steps / fused, 3),
-            "recompute_ms": round(tm["recompute_ms"], 2),
-            "adjoint_steps_ms": round(tm["adjoint_steps_ms"], 2)}
+    out = {"workload": w.name, "steps": steps, "precision": "f32",
+           "fused_forward_mlups": round(fused, 1)}
+    grads = {}
+    for mode, kw in (("tape", {"tape": "auto"}), ("checkpoints", {"tape": False})):
+        run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20, timing=True,
+                                    **kw)
+        tgt = run.forward(tt, phi0, steps, keep=False)[:7].sum(0).clone()
+        t0 = time.perf_counter()
+        a = run.forward(theta, phi0, steps)
+        t1 = time.perf_counter()
+        L, lam = run.loss(a, tgt)
+        t2 = time.perf_counter()
+        grads[mode] = run.backward(lam)
+        t3 = time.perf_counter()
+        adj = w.cells * steps / (t3 - t2) / 1e6
+        tm = run.last_timing
+        adj_steps = w.cells * steps / (tm["adjoint_steps_ms"] / 1e3) / 1e6
+        r = {"checkpoint_every": 1 if run.tape is not None else 20,
+             "tape_GB": round(run.tape_bytes(steps) / 1e9, 1) if run.tape is not None else 0,
+             "forward_mlups": round(w.cells * steps / (t1 - t0) / 1e6, 1),
+             "adjoint_mlups": round(adj, 1), "adjoint_vs_forward": round(adj / fused, 3),
+             "adjoint_step_mlups": round(adj_steps, 1),
+             "adjoint_step_vs_forward": round(adj_steps / fused, 3),
+             "recompute_ms": round(tm["recompute_ms"], 2),
+             "adjoint_steps_ms": round(tm["adjoint_steps_ms"], 2)}
+        if mode == "tape":
+            out.update({"loss": L, "grad": dict(zip(adjoint.THETA, grads[mode])),
+                        "adjoint_mlups": r["adjoint_mlups"],
+                        "adjoint_vs_forward": r["adjoint_vs_forward"]})
+        out[mode] = r
+        del run, a, lam
+        torch.cuda.empty_cache()
+    out["grads_equal"] = grads["tape"] == grads["checkpoints"]
+    return out
 
 
 def traffic_from_profile(dims):

@@ -12,10 +12,13 @@ alpha (mu = 4 beta phi (phi^2 - phi*^2) - kappa lap phi), tau_f its tauPhi,
 tau_g its tauRho, and gravity enters as 2 phi* gravity on phi > 0 cells
 (src/paramread.c:141-170, src/collision_f.cu:96).
 
-The forward pass keeps a checkpoint every ``checkpoint_every`` steps; the
-backward pass recomputes each segment from its checkpoint, then runs the
-adjoint steps back through it.  All arithmetic runs in libzsc's HIP kernels;
-torch provides the device buffers.
+The forward pass keeps either a tape, the input state and phi of every step
+(the backward pass then recomputes nothing: 108 B per cell and step at FP32,
+so 128^3 cells x 400 steps take 90 GB of HBM), or a checkpoint every
+``checkpoint_every`` steps, from which the backward pass recomputes each
+segment before it runs the adjoint steps back through it.  By default the
+tape is kept when it fits in half of the free device memory.  All arithmetic
+runs in libzsc's HIP kernels; torch provides the device buffers.
 """
 from __future__ import annotations
 
@@ -76,9 +79,21 @@ class AdjointRunner:
     """Owns a context and the device buffers of one differentiable run."""
 
     def __init__(self, dims, base: zsc.Params, precision: str = "f32", checkpoint_every: int = 20,
-                 gravity_on: bool = True, device: str = "cuda", timing: bool = False):
+                 gravity_on: bool = True, device: str = "cuda", timing: bool = False,
+                 tape="auto", tape_fraction: float = 0.5):
+        """tape: True keeps every step's input state and phi (no recomputation),
+        False a checkpoint every ``checkpoint_every`` steps, "auto" the tape
+        when it takes at most ``tape_fraction`` of the device memory that is
+        free (to the driver or in torch's cache) when the forward pass starts."""
         if precision not in ("f32", "f64"):
             raise ValueError("the adjoint runs at f32 or f64")
+        if tape not in (True, False, "auto"):
+            raise ValueError("tape is True, False or 'auto'")
+        if int(checkpoint_every) < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.tape_mode = tape
+        self.tape_fraction = float(tape_fraction)
+        self.tape = None
         self.dims = tuple(dims)
         nx, ny, nz = self.dims
         self.n = nx * ny * nz
@@ -121,6 +136,19 @@ class AdjointRunner:
     def _phi(self):
         return torch.empty(self.n, dtype=self.dtype, device=self.device)
 
+    def tape_bytes(self, nsteps: int) -> int:
+        """Device bytes of the tape of an nsteps run: B_0 .. B_{N-2} and the
+        phi of every step."""
+        esize = 4 if self.dtype == torch.float32 else 8
+        return (26 * max(nsteps - 1, 0) + nsteps) * self.n * esize
+
+    def _use_tape(self, nsteps: int) -> bool:
+        if self.tape_mode != "auto":
+            return bool(self.tape_mode)
+        free, _ = torch.cuda.mem_get_info(self.device)
+        cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+        return self.tape_bytes(nsteps) <= self.tape_fraction * (free + cached)
+
     def _forward(self, theta, phi0, nsteps, keep):
         # Steps run as fused launches on post-collision lattices:
         # B_0 = collide(A_0), B_n = fused(B_{n-1}), A_N = stream(B_{N-1}).
@@ -128,6 +156,7 @@ class AdjointRunner:
         self._set_theta(theta)
         self.phi0 = phi0.to(self.device, self.dtype).contiguous().view(-1)
         h = self.ctx._h
+        self.checkpoints, self.tape = {}, None
         a0 = self._lattice()
         check(lib().zsc_adj_init(h, _vp(self.phi0), _vp(a0)))
         self.checkpoints = {0: a0} if keep else {}
@@ -135,6 +164,22 @@ class AdjointRunner:
         if nsteps == 0:
             self.final = a0
             return a0
+        if keep and self._use_tape(nsteps):
+            # tape: step n's input state (A_0, then B_{n-1}) and phi(A_n)
+            states, phis = [a0], [self._phi()]
+            b = self._lattice()
+            check(lib().zsc_adj_collide(h, _vp(a0), _vp(b), _vp(phis[0])))
+            for s in range(1, nsteps):
+                nb, phi = self._lattice(), self._phi()
+                check(lib().zsc_adj_fused(h, _vp(b), _vp(nb), _vp(phi)))
+                states.append(b)
+                phis.append(phi)
+                b = nb
+            a = self._lattice()
+            check(lib().zsc_adj_stream(h, _vp(b), _vp(a)))
+            self.tape = (states, phis)
+            self.final = a
+            return a
         b, b2, phi = self._lattice(), self._lattice(), self._phi()
         check(lib().zsc_adj_collide(h, _vp(a0), _vp(b), _vp(phi)))
         for s in range(1, nsteps):
@@ -163,12 +208,35 @@ class AdjointRunner:
         return float(self.grads[14].item()), lam
 
     def backward(self, lam: torch.Tensor, scale: float = 1.0):
-        """dL/dtheta by recomputing each checkpoint segment; lam = dL/da_N."""
+        """dL/dtheta from the tape, or by recomputing each checkpoint segment;
+        lam = dL/da_N."""
         torch.cuda.current_stream().synchronize()
         with torch.cuda.stream(self.stream):
             g = self._backward(lam, scale)
         self.stream.synchronize()
         return g
+
+    def _recompute(self, s0, s1):
+        """The input state and phi_n of every step of the segment s0 .. s1-1,
+        recomputed from its checkpoint."""
+        h = self.ctx._h
+        states, phis = [], []
+        if s0 == 0:
+            b, phi = self._lattice(), self._phi()
+            check(lib().zsc_adj_collide(h, _vp(self.checkpoints[0]), _vp(b), _vp(phi)))
+            states.append(self.checkpoints[0])
+            phis.append(phi)
+            n0 = 1
+        else:
+            b = self.checkpoints[s0]
+            n0 = s0
+        for _ in range(n0, s1):
+            nb, phi = self._lattice(), self._phi()
+            check(lib().zsc_adj_fused(h, _vp(b), _vp(nb), _vp(phi)))
+            states.append(b)
+            phis.append(phi)
+            b = nb
+        return states, phis
 
     def _backward(self, lam, scale):
         h = self.ctx._h
@@ -176,32 +244,23 @@ class AdjointRunner:
         loss_slot = float(self.grads[14].item())
         self.grads.zero_()
         lam2 = self._lattice()
-        starts = sorted(self.checkpoints)
+        if self.tape is not None:
+            # every step's input is on the tape: one segment, nothing to recompute
+            segments = [(0, self.nsteps, self.tape)]
+        else:
+            starts = sorted(self.checkpoints)
+            segments = [(s0, starts[i + 1] if i + 1 < len(starts) else self.nsteps, None)
+                        for i, s0 in enumerate(starts)]
         marks = []
-        for i in range(len(starts) - 1, -1, -1):
+        for s0, s1, taped in reversed(segments):
             if self.timing:
                 ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
                 ev[0].record(self.stream)
                 marks.append(ev)
-            s0 = starts[i]
-            s1 = starts[i + 1] if i + 1 < len(starts) else self.nsteps
-            # recompute the segment: the input state of every step and phi_n
-            states, phis = [], []
-            if s0 == 0:
-                b, phi = self._lattice(), self._phi()
-                check(lib().zsc_adj_collide(h, _vp(self.checkpoints[0]), _vp(b), _vp(phi)))
-                states.append(self.checkpoints[0])
-                phis.append(phi)
-                n0 = 1
+            if taped is not None:
+                states, phis = taped
             else:
-                b = self.checkpoints[s0]
-                n0 = s0
-            for _ in range(n0, s1):
-                nb, phi = self._lattice(), self._phi()
-                check(lib().zsc_adj_fused(h, _vp(b), _vp(nb), _vp(phi)))
-                states.append(b)
-                phis.append(phi)
-                b = nb
+                states, phis = self._recompute(s0, s1)
             if self.timing:
                 marks[-1][1].record(self.stream)
             for n in range(s1 - 1, s0 - 1, -1):

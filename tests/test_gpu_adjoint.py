@@ -59,7 +59,8 @@ def test_step_dot_product(need_gpu):
     assert abs(lhs - rhs) <= 1e-7 * abs(rhs)
 
 
-def test_fp64_gradients_match_taped_cpu(need_gpu):
+@pytest.mark.parametrize("tape", [True, False])
+def test_fp64_gradients_match_taped_cpu(need_gpu, tape):
     import torch
     from oracle import taped
     dims = (32, 32, 32)
@@ -71,9 +72,10 @@ def test_fp64_gradients_match_taped_cpu(need_gpu):
     theta = adjoint.theta_of(p)
     # target from perturbed physics, as the 128^3 workload
     tt = [theta[0] * 1.2, theta[1] * 0.9, theta[2], theta[3], theta[4] * 1.5]
-    run = adjoint.AdjointRunner(dims, p, precision="f64", checkpoint_every=7)
+    run = adjoint.AdjointRunner(dims, p, precision="f64", checkpoint_every=7, tape=tape)
     tgt = run.forward(tt, torch.from_numpy(phi0), steps, keep=False)[:7].sum(0).clone()
     L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, steps)
+    assert (run.tape is not None) == tape
     # taped FP64 CPU reference (init inside the tape)
     th = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in theta]
     c = taped.coefficients(*th, phi_star=p.phi_star, gamma=p.gamma)
@@ -87,6 +89,31 @@ def test_fp64_gradients_match_taped_cpu(need_gpu):
     assert abs(L - float(loss)) <= 1e-10 * abs(float(loss))
     for a, r in zip(grads, ref):
         assert abs(a - r) <= 1e-8 * max(abs(r), 1e-30), (a, r)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_tape_equals_recomputation(need_gpu, precision):
+    """The tape holds the same states and phi_n that the checkpointed
+    backward pass recomputes (the same kernels on the same inputs), so both
+    give the same gradients and the same dL/da_0 bit for bit; segments of 4
+    with a ragged last one (11 steps)."""
+    import torch
+    dims = (14, 10, 8)
+    p, phi0 = small_problem(dims, seed=2)
+    theta = adjoint.theta_of(p)
+    tt = [theta[0] * 1.1, theta[1] * 0.95, theta[2], theta[3], theta[4] * 1.3]
+    out = {}
+    for tape in (True, False):
+        run = adjoint.AdjointRunner(dims, p, precision=precision, checkpoint_every=4, tape=tape)
+        tgt = run.forward(tt, torch.from_numpy(phi0), 11, keep=False)[:7].sum(0).clone()
+        L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, 11)
+        assert (run.tape is not None) == tape
+        assert run.tape is not None or sorted(run.checkpoints) == [0, 4, 8]
+        out[tape] = (L, grads, run.lam0.cpu())
+    print(precision, "tape", out[True][1], "\ncheckpoints", out[False][1], flush=True)
+    assert out[True][0] == out[False][0]
+    assert out[True][1] == out[False][1]
+    assert torch.equal(out[True][2], out[False][2])
 
 
 def _workload_fd(precision, gravity_on):
