@@ -45,11 +45,12 @@ def parse():
                     help="steps of the 128^3 adjoint timing (0 disables)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline (0 disables)")
-    ap.add_argument("--workloads", default="1,3,dropin",
+    ap.add_argument("--workloads", default="1,3,dropin,share",
                     help="extra workloads reported beside the headline: 1 = 256^3 FP64+FP32 "
                          "2000 steps, 3 = 1024^3 bf16 500 steps, dropin = the reference's "
-                         "7-kernel step through the drop-in entry points at 256^3 FP32 "
-                         "(empty disables)")
+                         "7-kernel step through the drop-in entry points at 256^3 FP32, "
+                         "share = one GPU's z-slab of the 1024^3 box (1024x1024x128) at "
+                         "FP32 and bf16 (empty disables)")
     ap.add_argument("--copy-probe", type=int, default=4096, metavar="MIB",
                     help="time a plain device copy of MIB MiB: the practical read+write "
                          "ceiling reported beside the roofline (0 disables)")
@@ -337,6 +338,23 @@ def workload_3(steps: int):
             "sum_rho_drift_rel": (s1["sum_rho"] - s0["sum_rho"]) / abs(s0["sum_rho"])}
 
 
+def workload_share(steps: int, ty, zchunk, flags):
+    """One GPU's share of the 1024^3 box split into eight z-slabs (1024 x
+    1024 x 128 and the bubbles inside it), FP32 and bf16 storage: MLUPS and
+    roofline fraction of the step at that shape (periodic in z; the halo
+    exchange that would join the slabs is not run)."""
+    from lbm_bench_gpu_amd import configs
+    w = configs.z_slab(3, 8)
+    p = w.params()
+    out = {"workload": w.name, "dims": list(w.dims), "bubbles": int(len(w.bubbles)),
+           "steps": steps}
+    for prec in ("f32", "bf16"):
+        r = time_precision(prec, w, p, steps, 5, ty, zchunk, flags)
+        out[prec] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 4),
+                     "roofline_frac": r["roofline"]["frac"], "finite": r["finite"]}
+    return out
+
+
 def cpu_baseline(seconds: float, w):
     """FP64 oracle (C, OpenMP on all host cores) on a bounded sample of the
     timed workload: a 32-plane z-slab of its initial state (nx x ny x 32,
@@ -464,6 +482,8 @@ def main():
             extra_w[wid] = workload_3(500)
         elif wid == "dropin":
             extra_w[wid] = dropin_sequence(200)
+        elif wid == "share":
+            extra_w[wid] = workload_share(200, a.ty, a.zchunk, a.flags)
         else:
             raise SystemExit(f"no extra workload {wid}")
     probe = copy_probe(a.copy_probe) if a.copy_probe > 0 else None

@@ -76,3 +76,28 @@ def workload(i: int) -> Workload:
         return Workload("zsc-128^3-8bubbles-adjoint", dims, physics(*dims),
                         _gen(4, 8, 12, 12, dims), 400, ("f32",))
     raise ValueError(f"no workload {i}")
+
+
+def z_slab(i: int, parts: int, rank: int = 0) -> Workload:
+    """Rank ``rank``'s z-slab of workload i cut into ``parts`` equal slabs
+    (the z split of src/vgrid.c:37-47): the box nx x ny x nz/parts and the
+    bubbles that lie wholly inside the slab (R + W from its faces), moved to
+    its z origin.  A periodic run of it is the step at one GPU's share of the
+    decomposed box, without the halo exchange that would join the slabs."""
+    w = workload(i)
+    nx, ny, nz = w.dims
+    if parts < 1 or nz % parts or not 0 <= rank < parts:
+        raise ValueError("parts must divide nz and 0 <= rank < parts")
+    nzs = nz // parts
+    z0 = rank * nzs
+    width = w.inputs.width
+    # z is 1-based: the slab holds planes z0 + 1 .. z0 + nzs
+    keep = [b for b in w.bubbles
+            if b[2] - b[3] - width >= z0 + 1 and b[2] + b[3] + width <= z0 + nzs]
+    b = np.array(keep, dtype=np.float64).reshape(-1, 4)
+    b[:, 2] -= z0
+    dims = (nx, ny, nzs)
+    inputs = physics(*dims, **{k: getattr(w.inputs, k) for k in
+                              ("rho_l", "rho_h", "tau_rho", "tau_phi", "width", "sigma", "gamma",
+                               "eo", "phi_star", "gravity")})
+    return Workload(f"{w.name}-zslab{rank}of{parts}", dims, inputs, b, w.steps, w.precisions)

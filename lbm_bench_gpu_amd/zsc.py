@@ -139,6 +139,19 @@ def bytes_per_cell(precision: str) -> float:
 
 
 # --------------------------------------------------------------- context ---
+def _release_torch_cache() -> bool:
+    """Returns torch's cached, unused device blocks to the driver; True when
+    there were any."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is None or not torch.cuda.is_initialized():
+        return False
+    if torch.cuda.memory_reserved() <= torch.cuda.memory_allocated():
+        return False
+    torch.cuda.empty_cache()
+    return True
+
+
 class Context:
     """One periodic box on the current HIP device (zsc_create)."""
 
@@ -148,7 +161,13 @@ class Context:
         self.precision = precision
         self._h = ctypes.c_void_p()
         s = None if stream is None else ctypes.c_void_p(int(stream))
-        check(lib().zsc_create(nx, ny, nz, _lib.PRECISIONS[precision], s, ctypes.byref(self._h)))
+        args = (nx, ny, nz, _lib.PRECISIONS[precision], s, ctypes.byref(self._h))
+        rc = lib().zsc_create(*args)
+        if rc == _lib.ZSC_ERR_ALLOC and _release_torch_cache():
+            # libzsc allocates with hipMalloc: blocks that torch has freed but
+            # keeps cached are not available to it until torch returns them
+            rc = lib().zsc_create(*args)
+        check(rc)
 
     def close(self):
         if self._h:

@@ -25,3 +25,14 @@ def gpu_available() -> bool:
 def need_gpu():
     if not gpu_available():
         pytest.skip("no HIP device")
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _release_device_cache():
+    """Large device buffers that a test module's torch tensors held (the
+    adjoint's tape) go back to the driver before the next module, whose
+    libzsc contexts allocate with hipMalloc."""
+    yield
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.empty_cache()

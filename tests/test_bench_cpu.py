@@ -4,6 +4,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, ROOT)
@@ -24,3 +26,19 @@ def test_traffic_summary_is_read_and_at_least_algorithmic():
     assert set(t) == {"f32", "bf16", "fp16"}
     assert t["f32"] >= 208 * cells and t["bf16"] >= 108 * cells and t["fp16"] >= 108 * cells
     assert bench.traffic_from_profile((7, 7, 7)) == {}
+
+
+def test_z_slab_is_one_share_of_the_box():
+    from lbm_bench_gpu_amd import configs
+    full = configs.workload(3)
+    seen = 0
+    for r in range(8):
+        w = configs.z_slab(3, 8, r)
+        assert w.dims == (1024, 1024, 128)
+        assert w.params().values == full.params().values
+        b = w.bubbles
+        assert (b[:, 2] - b[:, 3] - 4.0 >= 1).all() and (b[:, 2] + b[:, 3] + 4.0 <= 128).all()
+        seen += len(b)
+    assert 0 < seen <= len(full.bubbles)
+    with pytest.raises(ValueError):
+        configs.z_slab(3, 7)

@@ -14,6 +14,8 @@ depend on size.
 * 1024^3, 512 bubbles (seed 3), bf16 storage: a few steps of the
   two-cells-per-lane kernel at its 32-bit offset limit conserve total phi and
   mass.
+* 1024 x 1024 x 128, one GPU's z-slab of that box: FP32 and bf16 storage
+  against FP64 over 200 steps, as at 512^3.
 """
 import numpy as np
 import pytest
@@ -107,3 +109,30 @@ def test_1024_box_bf16_two_cells_per_lane(need_gpu):
           flush=True)
     assert abs(d_phi) <= 1e-6 and abs(d_rho) <= 1e-6
     assert abs(v - 1) < 1e-3
+
+
+@pytest.mark.parametrize("prec,tol_phi", [("f32", 1e-3), ("bf16", 1e-3)])
+def test_z_slab_share_vs_fp64(need_gpu, prec, tol_phi):
+    """The 1024^3 box's z-slab for one GPU of eight (1024 x 1024 x 128, the
+    bubbles wholly inside it): 200 steps at FP32 and bf16 storage against
+    FP64 on the same GPU; total phi and mass conserved to round-off."""
+    w = configs.z_slab(3, 8)
+    p = w.params()
+    out = {}
+    for pr in ("f64", prec):
+        with zsc.Context(*w.dims, precision=pr) as ctx:
+            ctx.set_params(p)
+            ctx.state_init_bubbles(w.bubbles)
+            s0 = ctx.stats()
+            ctx.step(200)
+            s1 = ctx.stats()
+            out[pr] = (ctx.moments()[0], s0, s1)
+    phi, s0, s1 = out[prec]
+    e_phi = float(np.abs(phi - out["f64"][0]).max())
+    d_phi = (s1["sum_phi"] - s0["sum_phi"]) / abs(s0["sum_phi"])
+    d_rho = (s1["sum_rho"] - s0["sum_rho"]) / abs(s0["sum_rho"])
+    print(f"{w.name} {prec}: max|dphi| {e_phi:.3e}, sum phi drift {d_phi:.2e}, "
+          f"sum rho drift {d_rho:.2e}", flush=True)
+    assert len(w.bubbles) > 0
+    assert e_phi <= tol_phi
+    assert abs(d_phi) <= 3e-6 and abs(d_rho) <= 3e-6
