@@ -124,17 +124,17 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     // FP32 gains from the register-held g prefetch (2 blocks per CU, 64
     // planes); the 16-bit lattices, bound by memory-level parallelism, from
     // two cells per lane (32-bit words: twice the bytes per load) where rows
-    // are a multiple of 128 cells, else from 6 waves per SIMD and the LDS
-    // stash of f_5 / f_6 (two fewer global loads per cell, 1-2% faster; the
-    // FP32 kernel, bound by its registers, runs 2% slower with it)
+    // are a multiple of 128 cells, else from 6 waves per SIMD; both from the
+    // stash of f_5 / f_6 (two fewer global loads per pulled cell, 1-2%
+    // faster; the FP32 kernel, bound by its registers, runs 2% slower with it)
     const bool wide = nx % 128 == 0;
     const int stash = kFusedNtStore | kFusedZStash;
     if (precision == ZSC_F32)
         c->cfg = {8, 64, kFusedNtStore | kFusedPrefetch};
     else if (precision == ZSC_BF16)
-        c->cfg = wide ? FusedCfg{4, 24, kFusedNtStore | kFusedWide} : FusedCfg{8, 32, stash};
+        c->cfg = wide ? FusedCfg{4, 24, stash | kFusedWide} : FusedCfg{8, 32, stash};
     else if (precision == ZSC_FP16)
-        c->cfg = wide ? FusedCfg{4, 24, kFusedNtStore | kFusedWide} : FusedCfg{8, 16, stash};
+        c->cfg = wide ? FusedCfg{4, 24, stash | kFusedWide} : FusedCfg{8, 16, stash};
     else
         c->cfg = {4, 16, kFusedNtStore};
     hipError_t e = hipGetDevice(&c->device);

@@ -987,7 +987,10 @@ struct RowStarts {
 // the operand order of pull_f32): f[j][i] for the cell x0 + 2l + j.
 template <class S, int FL>
 __device__ inline void pull_f_w(const Lattice<S> &B, int64_t n, const RowStarts &rs, uint32_t xo,
-                                uint32_t left, uint32_t right, const Coef<float> &c, float f[2][7])
+                                uint32_t left, uint32_t right, const Coef<float> &c, float f[2][7],
+                                typename Pair<typename Traits<S>::SF>::type *s5 = nullptr,
+                                typename Pair<typename Traits<S>::SF>::type *s6 = nullptr,
+                                bool use = false)
 {
     using SF = typename Traits<S>::SF;
     const f32x2 f0 = *(const f32x2 *)((const char *)B.f0 + (uint64_t)((rs.r[1][1] + xo) * 4u));
@@ -996,15 +999,24 @@ __device__ inline void pull_f_w(const Lattice<S> &B, int64_t n, const RowStarts 
 #pragma unroll
     for (int i = 1; i < 7; ++i) {
         const SF *p = B.f + (int64_t)(i - 1) * n;
-        const auto own = ld_pair(p, rs.r[1][1] + xo);
         const uint32_t row = rs.src(cy(i), cz(i));
-        typename Pair<SF>::type up;
-        if (cx(i) == 0)
+        // with a stash (s5, s6), f_5's upwind word is the previous plane's own
+        // word and f_6's own word the previous plane's upwind one (`use`)
+        typename Pair<SF>::type own, up;
+        if (s6 != nullptr && i == 6 && use)
+            own = *s6;
+        else
+            own = ld_pair(p, rs.r[1][1] + xo);
+        if (s5 != nullptr && i == 5 && use)
+            up = *s5;
+        else if (cx(i) == 0)
             up = ld_pair(p, row + xo);
         else if (cx(i) == 1)
             up = shift_word<1>(ld_pair(p, row + xo), ld_pair(p, row + left));
         else
             up = shift_word<-1>(ld_pair(p, row + xo), ld_pair(p, row + right));
+        if (s5 != nullptr && i == 5) *s5 = own;
+        if (s6 != nullptr && i == 6) *s6 = up;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
             f[j][i] = c.eta * Codec<SF>::dec0(elem<SF>(up, j)) +
@@ -1044,12 +1056,18 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY
     const uint32_t left = (uint32_t)(x0 == 0 ? nx - 2 : x0 - 2);
     const uint32_t right = (uint32_t)(x0 + 128 == nx ? 0 : x0 + 128);
 
-    auto plane_phi = [&](int zz) {
+    // f_5 / f_6 words shared by consecutive planes (own row, ring row)
+    constexpr bool kZS = (FL & kZStash) != 0;
+    using FW = typename Pair<SF>::type;
+    FW zs5, zs6, zr5, zr6;
+
+    auto plane_phi = [&](int zz, bool use) {
         const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
         const int slot = (zz + 4) & 3;
         if (own) {
             float f[2][7];
-            pull_f_w<S, FL>(in, n, RowStarts(y, zc, nx, ny, nz), xo, left, right, c, f);
+            pull_f_w<S, FL>(in, n, RowStarts(y, zc, nx, ny, nz), xo, left, right, c, f,
+                            kZS ? &zs5 : nullptr, kZS ? &zs6 : nullptr, use);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 sp[slot][ty + 1][j][tx + 1] = sum_f(f[j]);
@@ -1063,7 +1081,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY
             int hy = ty == 0 ? y0 - 1 : y0 + th;
             hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
             float f[2][7];
-            pull_f_w<S, FL>(in, n, RowStarts(hy, zc, nx, ny, nz), xo, left, right, c, f);
+            pull_f_w<S, FL>(in, n, RowStarts(hy, zc, nx, ny, nz), xo, left, right, c, f,
+                            kZS ? &zr5 : nullptr, kZS ? &zr6 : nullptr, use);
 #pragma unroll
             for (int j = 0; j < 2; ++j) sp[slot][r][j][tx + 1] = sum_f(f[j]);
         }
@@ -1084,8 +1103,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY
     };
 
     __builtin_amdgcn_s_setprio(2);
-    plane_phi(z0 - 1);
-    plane_phi(z0);
+    plane_phi(z0 - 1, false);
+    plane_phi(z0, true);
     for (int k = z0; k < z1; ++k) {
 #if ZSC_OPAQUE_STRIDE
         {
@@ -1108,7 +1127,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY
                 if (cx(i) == -1) ge[i] = ld_pair(p, row + right);
             }
         }
-        plane_phi(k + 1);
+        plane_phi(k + 1, true);
         __syncthreads();
         __builtin_amdgcn_s_setprio(0);
         if (own) {
@@ -1401,6 +1420,8 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
         if constexpr (TY == 4 || TY == 8) {
             // two cells per lane (rows of 128 cells)
             if ((cfg.flags & kFusedWide) && g.nx % 128 == 0) {
+                if ((cfg.flags & kFusedZStash) && (cfg.flags & kNtStore))
+                    return launch_fused_w<S, TY, kNtStore | kZStash>(g, c, s, cfg, Bin, Bout, phi);
                 if (cfg.flags & kNtStore) return launch_fused_w<S, TY, 1>(g, c, s, cfg, Bin, Bout, phi);
                 return launch_fused_w<S, TY, 0>(g, c, s, cfg, Bin, Bout, phi);
             }

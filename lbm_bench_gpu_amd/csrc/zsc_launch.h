@@ -43,7 +43,7 @@ enum : int {
     kFusedPrefetch = 4,     // load (part of) the next plane's g during this plane
     kFusedIdentityMap = 8,  // no XCD-contiguous tile remapping
     kFusedWide = 16,        // two cells per lane (nx % 128 == 0, ty 4 or 8; not FP64)
-    kFusedZStash = 32,      // f_5 / f_6 shared by consecutive planes kept in LDS
+    kFusedZStash = 32,      // f_5 / f_6 shared by consecutive planes kept (LDS or registers)
 };
 struct FusedCfg {
     int ty;      // tile rows (waves per block); tile is 64 x ty cells
