@@ -149,7 +149,8 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     HBM (nothing to recompute; 90 GB at 400 steps) and once with checkpoints
     every 20 steps (the recomputation of every segment included), and of
     the adjoint steps alone (HIP events around each segment's recomputation
-    and its backward steps).  Both modes give the same gradients."""
+    and its backward steps), after one untimed backward pass.  Both modes
+    give the same gradients."""
     import torch
     from lbm_bench_gpu_amd import adjoint, configs, zsc
     w = configs.workload(4)
@@ -176,6 +177,7 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
         a = run.forward(theta, phi0, steps)
         t1 = time.perf_counter()
         L, lam = run.loss(a, tgt)
+        run.backward(lam.clone())  # warm-up (kernel loading, first-touch buffers)
         t2 = time.perf_counter()
         grads[mode] = run.backward(lam)
         t3 = time.perf_counter()

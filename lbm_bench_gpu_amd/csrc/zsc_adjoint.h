@@ -244,6 +244,21 @@ inline int adj_blocks(int64_t n)
     return (int)(b < 1 ? 1 : (b > kAdjMaxBlocks ? kAdjMaxBlocks : b));
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs.  With `remap` the
+// grid-stride loop gives each XCD a contiguous eighth of every sweep, so the
+// neighbouring rows and planes that a cell's pulls and stencils read were
+// mostly fetched into the same XCD's L2 (the per-block partial sums stay
+// indexed by blockIdx).
+__device__ inline int64_t adj_first(int remap)
+{
+    int64_t b = blockIdx.x;
+    const int64_t nb = gridDim.x;
+    if (remap && (nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    return b * blockDim.x + threadIdx.x;
+}
+#define ZSC_ADJ_STRIDE(idx, n, remap) \
+    for (int64_t idx = adj_first(remap); idx < (n); idx += (int64_t)gridDim.x * blockDim.x)
+
 // Pass 1: lam_out (local part, including phi's local uses), the stencil-
 // adjoint fields a_lap, a_grad (4 fields) and per-block coefficient
 // partials.  A and lam are plain (26, n) lattices.
@@ -251,12 +266,12 @@ template <class R, bool PULL>
 __global__ void __launch_bounds__(256)
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
                 const R *__restrict__ phi, const R *__restrict__ lam, R *__restrict__ lam_out,
-                R *__restrict__ fields, double *partial, int accumulate)
+                R *__restrict__ fields, double *partial, int accumulate, int remap)
 {
     R ap[kNumGrad];
 #pragma unroll
     for (int j = 0; j < kNumGrad; ++j) ap[j] = R(0);
-    ZSC_GRID_STRIDE(idx, n)
+    ZSC_ADJ_STRIDE(idx, n, remap)
     {
         int x, y, z;
         cell_xyz(idx, nx, ny, x, y, z);
@@ -316,9 +331,9 @@ __global__ void __launch_bounds__(256)
 template <class R>
 __global__ void __launch_bounds__(256)
     k_adj_pass2(int nx, int ny, int nz, int64_t n, const R *__restrict__ fields,
-                R *__restrict__ lam_out)
+                R *__restrict__ lam_out, int remap)
 {
-    ZSC_GRID_STRIDE(idx, n)
+    ZSC_ADJ_STRIDE(idx, n, remap)
     {
         int x, y, z;
         cell_xyz(idx, nx, ny, x, y, z);
@@ -451,14 +466,15 @@ struct Adj {
         const int nb = adj_blocks(g.n);
         double *partial = partials(g, scratch, grads ? 0 : 1);
         const Coef<R> c = make_coef<R>(p, grav_on);
+        const int remap = 1;
         if (pulled)
             hipLaunchKernelGGL((k_adj_pass1<R, true>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
-                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc);
+                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
         else
             hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
-                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc);
+                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
         hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
-                           (const R *)fields, lam_out);
+                           (const R *)fields, lam_out, remap);
         if (grads)
             hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();
