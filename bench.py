@@ -45,6 +45,9 @@ def parse():
                     help="steps of the 128^3 adjoint timing (0 disables)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline (0 disables)")
+    ap.add_argument("--placement-trials", type=int, default=6,
+                    help="lattice allocations tried per timed context, the fastest kept "
+                         "(zsc_place_lattices; 0 disables)")
     ap.add_argument("--workloads", default="1,3,dropin,share",
                     help="extra workloads reported beside the headline: 1 = 256^3 FP64+FP32 "
                          "2000 steps, 3 = 1024^3 bf16 500 steps, dropin = the reference's "
@@ -63,14 +66,19 @@ def configure(ctx, ty, zchunk, flags):
                              flags if flags >= 0 else 1)
 
 
-def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags):
+def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags, trials=0):
+    """One context: `trials` lattice placements (zsc_place_lattices; 0 for
+    none), the state, `warmup` untimed steps, then `steps` timed ones."""
     import torch
     from lbm_bench_gpu_amd import zsc
     nx, ny, nz = w.dims
     ctx = zsc.Context(nx, ny, nz, precision=prec)
+    placement = None
     try:
         configure(ctx, ty, zchunk, flags)
         ctx.set_params(params)
+        if trials > 0:
+            placement = [round(m, 4) for m in ctx.place_lattices(trials)]
         ctx.state_init_bubbles(w.bubbles)
         ctx.step(warmup + 1)  # the first step also runs the A-lattice prologue
         ctx.sync()
@@ -95,6 +103,7 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags):
                      "traffic": None, "bytes_per_cell": bpc},
         "sum_phi": st["sum_phi"],
         "finite": bool(np.isfinite(st["sum_phi"])),
+        "placement_ms": placement,
     }
 
 
@@ -340,7 +349,7 @@ def workload_3(steps: int):
             "sum_rho_drift_rel": (s1["sum_rho"] - s0["sum_rho"]) / abs(s0["sum_rho"])}
 
 
-def workload_share(steps: int, ty, zchunk, flags):
+def workload_share(steps: int, ty, zchunk, flags, trials):
     """One GPU's share of the 1024^3 box split into eight z-slabs (1024 x
     1024 x 128 and the bubbles inside it), FP32 and bf16 storage: MLUPS and
     roofline fraction of the step at that shape (periodic in z; the halo
@@ -351,9 +360,10 @@ def workload_share(steps: int, ty, zchunk, flags):
     out = {"workload": w.name, "dims": list(w.dims), "bubbles": int(len(w.bubbles)),
            "steps": steps}
     for prec in ("f32", "bf16"):
-        r = time_precision(prec, w, p, steps, 5, ty, zchunk, flags)
+        r = time_precision(prec, w, p, steps, 5, ty, zchunk, flags, trials)
         out[prec] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 4),
-                     "roofline_frac": r["roofline"]["frac"], "finite": r["finite"]}
+                     "roofline_frac": r["roofline"]["frac"], "finite": r["finite"],
+                     "placement_ms": r["placement_ms"]}
     return out
 
 
@@ -462,12 +472,14 @@ def main():
                      a.steps, ("f32",))
     params = w.params()
     print(f"# workload {w.name}: fp32 {a.steps} steps", file=sys.stderr, flush=True)
-    main_r = time_precision("f32", w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags)
+    main_r = time_precision("f32", w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags,
+                            a.placement_trials)
     extra = {}
     variants = [s for s in a.variants.split(",") if s]
     for v in variants:
         print(f"# {v} storage", file=sys.stderr, flush=True)
-        extra[v] = time_precision(v, w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags)
+        extra[v] = time_precision(v, w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags,
+                                  a.placement_trials)
     acc = None
     if a.accuracy_steps > 0:
         acc = accuracy(w, params, a.accuracy_steps, ["f32"] + variants, a.ty, a.zchunk, a.flags)
@@ -485,7 +497,7 @@ def main():
         elif wid == "dropin":
             extra_w[wid] = dropin_sequence(200)
         elif wid == "share":
-            extra_w[wid] = workload_share(200, a.ty, a.zchunk, a.flags)
+            extra_w[wid] = workload_share(200, a.ty, a.zchunk, a.flags, a.placement_trials)
         else:
             raise SystemExit(f"no extra workload {wid}")
     probe = copy_probe(a.copy_probe) if a.copy_probe > 0 else None
@@ -515,9 +527,11 @@ def main():
                    "precision": "fp32 storage, fp32 compute", "gravity": "on"},
         "roofline": main_r["roofline"],
         "kernel_ms": round(main_r["kernel_ms"], 4),
+        "placement_ms": main_r["placement_ms"],
         "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                          "speedup_vs_fp32": round(v["mlups"] / main_r["mlups"], 3),
-                         "roofline": v["roofline"]} for k, v in extra.items()},
+                         "roofline": v["roofline"], "placement_ms": v["placement_ms"]}
+                     for k, v in extra.items()},
         "accuracy_vs_fp64": acc,
         "adjoint": adj,
         "workloads": extra_w,

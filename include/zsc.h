@@ -191,6 +191,16 @@ int zsc_step(zsc_ctx *ctx, int nsteps);
 /* Same, and returns the device time of the launches between two HIP events
  * on the context stream (synchronises). */
 int zsc_step_timed(zsc_ctx *ctx, int nsteps, double *ms);
+/* Allocates the two lattices up to `trials` (1..16) times, times four fused
+ * steps on each pair (from a rest state, with the context's parameters and
+ * launch shape) and keeps the fastest pair: the step's time depends on the
+ * physical pages behind the lattices, which an allocation cannot choose.
+ * ms (may be NULL) receives each trial's time per step, 0 for trials not run
+ * (a further pair must leave a quarter of the free device memory).  Call it
+ * after zsc_set_params and before the state is set (ZSC_ERR_STATE
+ * otherwise); the state set afterwards is unaffected.  No reference
+ * counterpart. */
+int zsc_place_lattices(zsc_ctx *ctx, int trials, double *ms);
 /* Post-stream (A) populations as FP64 host arrays, pop-major: f[i*N + idx],
  * g[i*N + idx].  Storage precision is decoded. */
 int zsc_state_get(zsc_ctx *ctx, double *f_host, double *g_host);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "zsc_state_init_bubbles": (_I, [_VP, _DP, _I]),
     "zsc_step": (_I, [_VP, _I]),
     "zsc_step_timed": (_I, [_VP, _I, _DP]),
+    "zsc_place_lattices": (_I, [_VP, _I, _DP]),
     "zsc_state_get": (_I, [_VP, _DP, _DP]),
     "zsc_state_set": (_I, [_VP, _DP, _DP]),
     "zsc_state_moments": (_I, [_VP, _DP, _DP, _DP, _DP, _DP]),

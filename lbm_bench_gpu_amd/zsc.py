@@ -259,6 +259,13 @@ class Context:
         check(lib().zsc_step_timed(self._h, int(nsteps), ctypes.byref(ms)))
         return ms.value
 
+    def place_lattices(self, trials: int = 4) -> list:
+        """zsc_place_lattices: keeps the fastest of up to `trials` lattice
+        allocations; returns each trial's ms per step (0: not run)."""
+        ms = np.zeros(int(trials))
+        check(lib().zsc_place_lattices(self._h, int(trials), _dptr(ms)))
+        return [float(x) for x in ms]
+
     def state_get(self):
         nx, ny, nz = self.dims
         f = np.empty((7, nz, ny, nx))

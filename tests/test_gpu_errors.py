@@ -74,3 +74,22 @@ def test_diverged_state_is_reported(need_gpu, prec):
         ctx.state_init(phi0)                                      # recovers
         ctx.step(1)
         assert np.isfinite(ctx.stats()["sum_rho"])
+
+
+def test_place_lattices_preconditions(need_gpu):
+    dims = (16, 8, 4)
+    p, phi0 = small_case(dims, seed=34)
+    with zsc.Context(*dims, precision="f32") as ctx:
+        assert _code(ctx.place_lattices, 2) == ZSC_ERR_STATE          # no parameters yet
+        ctx.set_params(p)
+        assert _code(ctx.place_lattices, 0) == ZSC_ERR_ARG
+        assert _code(ctx.place_lattices, 17) == ZSC_ERR_ARG
+        ms = ctx.place_lattices(2)
+        assert ms[0] > 0.0 and len(ms) == 2
+        assert _code(ctx.place_lattices, 2) == ZSC_ERR_STATE          # lattices exist
+        ctx.state_init(phi0)
+        ctx.step(2)
+    with zsc.Context(*dims, precision="f32") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        assert _code(ctx.place_lattices, 2) == ZSC_ERR_STATE          # state already set

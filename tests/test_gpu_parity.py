@@ -366,3 +366,25 @@ def test_fp32_at_the_offset_limit(need_gpu):
         with pytest.raises(ZscError) as e:
             ctx.step(1)  # refused before any allocation or state is needed
         assert e.value.code in (ZSC_ERR_ARG,)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+def test_placed_lattices_give_the_same_steps(need_gpu, prec):
+    """zsc_place_lattices only chooses which allocation holds the lattices:
+    the state set and stepped afterwards equals an unplaced context's bit for
+    bit (same kernels, same inputs), in a box whose rows are a multiple of
+    128 cells (two-cell kernel at 16-bit storage)."""
+    dims = (128, 12, 10)
+    p, phi0 = small_case(dims, nb=3, seed=29)
+    out = []
+    for trials in (0, 3):
+        with zsc.Context(*dims, precision=prec) as ctx:
+            ctx.set_params(p)
+            if trials:
+                ms = ctx.place_lattices(trials)
+                print(prec, "placement trials (ms per step)", ms, flush=True)
+                assert all(m > 0 for m in ms)
+            ctx.state_init(phi0)
+            ctx.step(7)
+            out.append(ctx.state_get())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
