@@ -142,14 +142,28 @@ def _workload_fd(precision, gravity_on):
     return grads, fd
 
 
-def test_adjoint_workload_smooth_vs_central_differences(need_gpu):
-    """The 128^3 workload (8 bubbles R = 12, seed 4, 400 steps, checkpoints
-    every 20, target from kappa x1.2, beta x0.9) with gravity switched off, so
-    the loss is smooth in every parameter: FP64 adjoint vs central
-    differences (h = 1e-4 relative) within 1e-6; the gravity derivative is 0."""
-    grads, fd = _workload_fd("f64", False)
-    for name, a, r in list(zip(adjoint.THETA, grads, fd))[:4]:
-        assert abs(a - r) <= 1e-6 * abs(r), (name, a, r)
+SMOOTH_TOL = {"f64": {"kappa": 1e-6, "beta": 1e-6, "tau_f": 1e-6, "tau_g": 1e-6},
+              "f32": {"kappa": 1e-3, "beta": 1e-3, "tau_f": 2e-2, "tau_g": 5e-2}}
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
+    """The 128^3 workload (8 bubbles R = 12, seed 4, 400 steps, target from
+    kappa x1.2, beta x0.9) with gravity switched off, so the loss is smooth
+    in every parameter: the adjoint against FP64 central differences (h =
+    1e-4 relative).  FP64: 1.8e-8 (kappa), 2.0e-9 (beta), 2.7e-8 (tau_f),
+    3.0e-7 (tau_g), bound 1e-6.  FP32 differentiates its own trajectory and
+    propagates the adjoint in FP32: 2.0e-4, 3.6e-4, 6.5e-3, 1.8e-2.  Against
+    central differences of the FP32 loss itself (h = 1e-2 .. 3e-3, where
+    FP32 noise allows) kappa, beta and tau_f agree to 1e-4 .. 2.5e-6, so
+    their distance from FP64 is the FP32 trajectory's; tau_g's derivative,
+    1.13 against O(1e4 .. 1e5) for the others, keeps ~1.5% of FP32 adjoint
+    rounding (FP64 coefficient sums remove only a fifth of it).  The bounds
+    are those measurements with room; the gravity derivative is 0."""
+    grads, fd = _workload_fd(precision, False)
+    errs = {name: abs(a - r) / abs(r) for name, a, r in list(zip(adjoint.THETA, grads, fd))[:4]}
+    print(precision, "relative error vs FD", errs, flush=True)
+    assert all(errs[k] <= SMOOTH_TOL[precision][k] for k in errs), errs
     assert grads[4] == 0.0
 
 
