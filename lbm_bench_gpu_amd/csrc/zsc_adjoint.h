@@ -65,7 +65,9 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     const R jx = ga[1] - ga[2] + ga[7] - ga[8] + ga[9] - ga[10] + ga[11] - ga[12] + ga[13] - ga[14];
     const R jy = ga[3] - ga[4] + ga[7] - ga[8] - ga[9] + ga[10] + ga[15] - ga[16] + ga[17] - ga[18];
     const R jz = ga[5] - ga[6] + ga[11] - ga[12] - ga[13] + ga[14] + ga[15] - ga[16] - ga[17] + ga[18];
-    const R u[3] = {jx / rho, jy / rho, jz / rho};
+    // quotients by rho: FP32 by the hardware reciprocal, FP64 exact
+    using Qr = ::zsc::Q<R>;
+    const R u[3] = {Qr::div(jx, rho), Qr::div(jy, rho), Qr::div(jz, rho)};
     const R lap = lap19(p);
     const R G[3] = {
         (R(2) * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] + p[13] - p[14]) *
@@ -79,7 +81,7 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     const R H = phi > R(0) ? R(1) : R(0);
     R F[3] = {mu * G[0], mu * G[1], mu * G[2] + H * c.grav};
     R v[3];
-    for (int a = 0; a < 3; ++a) v[a] = u[a] + R(0.5) * F[a] / rho;
+    for (int a = 0; a < 3; ++a) v[a] = u[a] + Qr::div(R(0.5) * F[a], rho);
     const R hg = R(0.5) * c.gamma * c.inv_tau_phi;
     const R cfk = c.inv_tau_phi * c.inv_eta2;
     const R Af = hg * mu, Cf = cfk * phi;
@@ -184,8 +186,8 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     R a_u[3];
     for (int a = 0; a < 3; ++a) {
         a_u[a] = a_v[a];
-        a_F[a] += a_v[a] * R(0.5) / rho;
-        a_rho += -F[a] * a_v[a] * R(0.5) / (rho * rho);
+        a_F[a] += Qr::div(a_v[a] * R(0.5), rho);
+        a_rho += Qr::div(-F[a] * a_v[a] * R(0.5), rho * rho);
     }
     // F = mu grad(phi) + grav H(phi) e_z
     a_mu += G[0] * a_F[0] + G[1] * a_F[1] + G[2] * a_F[2];
@@ -199,8 +201,8 @@ __device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const
     // u = j / rho, rho = sum g, j = sum c g
     R a_j[3];
     for (int a = 0; a < 3; ++a) {
-        a_j[a] = a_u[a] / rho;
-        a_rho += -u[a] * a_u[a] / rho;
+        a_j[a] = Qr::div(a_u[a], rho);
+        a_rho += Qr::div(-u[a] * a_u[a], rho);
     }
 #pragma unroll
     for (int i = 0; i < 7; ++i) afa[i] = tf * lf[i];
@@ -234,14 +236,21 @@ __device__ inline void block_reduce_grads(double *ap, double *partial, bool accu
     }
 }
 
-// Blocks of the adjoint kernels: one cell per thread up to 16.7 M cells,
-// then a few per thread (per-thread sums at precision R, block and global
-// sums in FP64).  The partial buffers hold kAdjMaxBlocks rows.
+// Blocks of the adjoint kernels: kAdjCellsPerThread cells per thread (the
+// per-block reduction of the coefficient sums, ~15% of pass 1's VALU work
+// for one cell, is paid once per thread; 2 times 3.6% faster than 1 and
+// 1.5% faster than 4, whose last round of blocks runs part-empty), a
+// multiple of 8 blocks for the XCD-contiguous order, more cells per thread
+// beyond kAdjMaxBlocks blocks (per-thread sums at precision R, block and
+// global sums in FP64).  The partial buffers hold kAdjMaxBlocks rows.
 constexpr int kAdjMaxBlocks = 65536;
+constexpr int kAdjCellsPerThread = 2;
 inline int adj_blocks(int64_t n)
 {
-    const int64_t b = (n + 255) / 256;
-    return (int)(b < 1 ? 1 : (b > kAdjMaxBlocks ? kAdjMaxBlocks : b));
+    constexpr int cpt = kAdjCellsPerThread;
+    int64_t b = (n + 256 * cpt - 1) / (256 * cpt);
+    b = (b + 7) & ~(int64_t)7;
+    return (int)(b < 8 ? 8 : (b > kAdjMaxBlocks ? kAdjMaxBlocks : b));
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs.  With `remap` the
