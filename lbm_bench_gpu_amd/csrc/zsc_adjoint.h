@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(256)
     ZSC_ADJ_STRIDE(idx, n, remap)
     {
         int x, y, z;
-        cell_xyz(idx, nx, ny, x, y, z);
+        cell_xyz32((uint32_t)idx, nx, ny, x, y, z);
         // 32-bit element offsets (every (26, n) row is < 4 GiB, checked at
         // launch): scalar-base + vector-offset loads, fewer index registers
         const Nbr32 nb(x, y, z, nx, ny, nz);
@@ -345,20 +345,29 @@ __global__ void __launch_bounds__(256)
     ZSC_ADJ_STRIDE(idx, n, remap)
     {
         int x, y, z;
-        cell_xyz(idx, nx, ny, x, y, z);
-        Nbr nb(x, y, z, nx, ny, nz);
+        cell_xyz32((uint32_t)idx, nx, ny, x, y, z);
+        // 32-bit element offsets, scalar bases (rows < 4 GiB, checked at launch)
+        const Nbr32 nb(x, y, z, nx, ny, nz);
+        auto gather = [&](const R *f, R *h) {
+#pragma unroll
+            for (int d = 0; d < 19; ++d) h[d] = ld32z<0>(f, nb.at(cx(d), cy(d), cz(d)));
+        };
         R h[19];
-        gather19(fields, nb, h);
+        gather(fields, h);
         R a = lap19(h);
-        gather19(fields + n, nb, h);
+        gather(fields + n, h);
         R dv = R(2) * (h[1] - h[2]) + h[7] - h[8] + h[9] - h[10] + h[11] - h[12] + h[13] - h[14];
-        gather19(fields + 2 * n, nb, h);
+        gather(fields + 2 * n, h);
         dv += R(2) * (h[3] - h[4]) + h[7] - h[8] + h[10] - h[9] + h[15] - h[16] + h[17] - h[18];
-        gather19(fields + 3 * n, nb, h);
+        gather(fields + 3 * n, h);
         dv += R(2) * (h[5] - h[6]) + h[11] - h[12] + h[14] - h[13] + h[15] - h[16] + h[18] - h[17];
         a -= dv * W<R>::inv12;
+        const uint32_t own = (uint32_t)idx;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) lam_out[(int64_t)i * n + idx] += a;
+        for (int i = 0; i < 7; ++i) {
+            R *li = lam_out + (int64_t)i * n;
+            st32z<0>(li, own, ld32z<0>(li, own) + a);
+        }
     }
 }
 

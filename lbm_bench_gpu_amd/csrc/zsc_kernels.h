@@ -107,6 +107,17 @@ __device__ inline void cell_xyz(int64_t idx, int nx, int ny, int &x, int &y, int
     z = (int)(t / ny);
 }
 
+// Same for cell indices below 2^32 (the fused and adjoint kernels' range):
+// 32-bit divisions instead of 64-bit ones.
+__device__ inline void cell_xyz32(uint32_t idx, int nx, int ny, int &x, int &y, int &z)
+{
+    const uint32_t t = idx / (uint32_t)nx;
+    x = (int)(idx - t * (uint32_t)nx);
+    const uint32_t zz = t / (uint32_t)ny;
+    y = (int)(t - zz * (uint32_t)ny);
+    z = (int)zz;
+}
+
 #define ZSC_GRID_STRIDE(idx, n) \
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (n); \
          idx += (int64_t)gridDim.x * blockDim.x)
