@@ -202,7 +202,9 @@ int zsc_step_timed(zsc_ctx *ctx, int nsteps, double *ms);
  * counterpart. */
 int zsc_place_lattices(zsc_ctx *ctx, int trials, double *ms);
 /* Post-stream (A) populations as FP64 host arrays, pop-major: f[i*N + idx],
- * g[i*N + idx].  Storage precision is decoded. */
+ * g[i*N + idx].  Storage precision is decoded.  No reference counterpart
+ * (the reference keeps f and g in its own device arrays); with
+ * zsc_state_set, the state transfer of the fast path. */
 int zsc_state_get(zsc_ctx *ctx, double *f_host, double *g_host);
 /* Replaces the state by the given A populations (FP64 host, pop-major). */
 int zsc_state_set(zsc_ctx *ctx, const double *f_host, const double *g_host);
@@ -211,7 +213,9 @@ int zsc_state_set(zsc_ctx *ctx, const double *f_host, const double *g_host);
 int zsc_state_moments(zsc_ctx *ctx, double *phi, double *rho, double *ux,
                       double *uy, double *uz);
 /* Copies the A state into / from reference-style device arrays (A half of
- * each 2N array; F64 and F32 contexts only). */
+ * each 2N array, the f/g layout of src/main.cu:225-251; F64 and F32
+ * contexts only): moves a state between the drop-in kernels and the fused
+ * path.  No reference counterpart. */
 int zsc_state_export(zsc_ctx *ctx, void *const f[7], void *const g[19]);
 int zsc_state_import(zsc_ctx *ctx, void *const f[7], void *const g[19]);
 
@@ -310,7 +314,8 @@ int zsc_adj_init_backward(zsc_ctx *ctx, const void *phi0, const void *lam0, void
                           double *grads);
 
 /* Bytes per lattice update the fused step must move (26 populations read
- * and written once at the storage precision). */
+ * and written once at the storage precision).  No reference counterpart:
+ * roofline accounting. */
 double zsc_bytes_per_cell(int precision);
 
 #ifdef __cplusplus
