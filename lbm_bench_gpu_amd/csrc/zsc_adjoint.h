@@ -336,37 +336,103 @@ __global__ void __launch_bounds__(256)
 }
 
 // Pass 2: a_phi's stencil part lap[a_lap] - div[a_grad] (the transposed
-// stencils), added to the adjoint of every f population.
-template <class R>
-__global__ void __launch_bounds__(256)
-    k_adj_pass2(int nx, int ny, int nz, int64_t n, const R *__restrict__ fields,
-                R *__restrict__ lam_out, int remap)
+// stencils), added to the adjoint of every f population.  Tiles of 64 x TY
+// cells march along z; the four stencil fields of planes k-1, k, k+1 (tile
+// and 1-cell ring) sit in a 4-slot LDS ring, one barrier per plane, so each
+// field value is loaded from memory about once instead of once per stencil
+// point.  Row offsets are 32-bit (rows < 4 GiB, checked at launch).
+template <class R, int TY>
+__global__ void __launch_bounds__(64 * TY)
+    k_adj_pass2(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk,
+                int remap, const R *__restrict__ fields, R *__restrict__ lam_out)
 {
-    ZSC_ADJ_STRIDE(idx, n, remap)
-    {
-        int x, y, z;
-        cell_xyz32((uint32_t)idx, nx, ny, x, y, z);
-        // 32-bit element offsets, scalar bases (rows < 4 GiB, checked at launch)
-        const Nbr32 nb(x, y, z, nx, ny, nz);
-        auto gather = [&](const R *f, R *h) {
+    __shared__ R sh[4][4][TY + 2][66];  // [slot][field][row][column]
+    const int b = blockIdx.x;
+    int l = b;
+    if (remap && (nblk & 7) == 0) l = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
+    const int x0 = bx * 64, y0 = by * TY, z0 = bz * zchunk;
+    const int z1 = min(nz, z0 + zchunk);
+    const int tw = min(64, nx - x0), th = min(TY, ny - y0);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const bool own = tx < tw && ty < th;
+    const int x = x0 + min(tx, tw - 1), y = y0 + min(ty, th - 1);
+    const int nh = 2 * tw + 2 * (th + 2);  // ring cells per plane
+    const int tid = ty * 64 + tx;
+    const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
+
+    // the four fields of plane zz (tile + ring) into its slot
+    auto load_plane = [&](int zz) {
+        const int zc = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        const int slot = (zz + 4) & 3;
+        const uint32_t pz = (uint32_t)zc * nxy;
+        if (own) {
+            const uint32_t o = pz + (uint32_t)y * (uint32_t)nx + (uint32_t)x;
 #pragma unroll
-            for (int d = 0; d < 19; ++d) h[d] = ld32z<0>(f, nb.at(cx(d), cy(d), cz(d)));
-        };
-        R h[19];
-        gather(fields, h);
-        R a = lap19(h);
-        gather(fields + n, h);
-        R dv = R(2) * (h[1] - h[2]) + h[7] - h[8] + h[9] - h[10] + h[11] - h[12] + h[13] - h[14];
-        gather(fields + 2 * n, h);
-        dv += R(2) * (h[3] - h[4]) + h[7] - h[8] + h[10] - h[9] + h[15] - h[16] + h[17] - h[18];
-        gather(fields + 3 * n, h);
-        dv += R(2) * (h[5] - h[6]) + h[11] - h[12] + h[14] - h[13] + h[15] - h[16] + h[18] - h[17];
-        a -= dv * W<R>::inv12;
-        const uint32_t own = (uint32_t)idx;
+            for (int q = 0; q < 4; ++q) sh[slot][q][ty + 1][tx + 1] = ld32z<0>(fields + q * n, o);
+        }
+        for (int h = tid; h < nh; h += 64 * TY) {
+            int hx, hy, r, col;
+            if (h < tw) {  // row below the tile
+                col = h + 1;
+                r = 0;
+                hx = x0 + h;
+                hy = y0 - 1;
+            } else if (h < 2 * tw) {  // row above
+                col = h - tw + 1;
+                r = th + 1;
+                hx = x0 + h - tw;
+                hy = y0 + th;
+            } else if (h < 2 * tw + th + 2) {  // column left, corners included
+                col = 0;
+                r = h - 2 * tw;
+                hx = x0 - 1;
+                hy = y0 - 1 + r;
+            } else {  // column right
+                col = tw + 1;
+                r = h - 2 * tw - th - 2;
+                hx = x0 + tw;
+                hy = y0 - 1 + r;
+            }
+            hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
+            hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+            const uint32_t o = pz + (uint32_t)hy * (uint32_t)nx + (uint32_t)hx;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) {
-            R *li = lam_out + (int64_t)i * n;
-            st32z<0>(li, own, ld32z<0>(li, own) + a);
+            for (int q = 0; q < 4; ++q) sh[slot][q][r][col] = ld32z<0>(fields + q * n, o);
+        }
+    };
+
+    load_plane(z0 - 1);
+    load_plane(z0);
+    for (int k = z0; k < z1; ++k) {
+        const uint32_t o = (uint32_t)k * nxy + (uint32_t)y * (uint32_t)nx + (uint32_t)x;
+        R lf[7];
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < 7; ++i) lf[i] = ld32z<0>(lam_out + (int64_t)i * n, o);
+        }
+        load_plane(k + 1);
+        __syncthreads();
+        if (own) {
+            // slots of planes k-1, k, k+1
+            const int sl[3] = {(k + 3) & 3, (k + 4) & 3, (k + 5) & 3};
+            auto gather = [&](int q, R *h) {
+#pragma unroll
+                for (int d = 0; d < 19; ++d)
+                    h[d] = sh[sl[1 + cz(d)]][q][ty + 1 + cy(d)][tx + 1 + cx(d)];
+            };
+            R h[19];
+            gather(0, h);
+            R a = lap19(h);
+            gather(1, h);
+            R dv = R(2) * (h[1] - h[2]) + h[7] - h[8] + h[9] - h[10] + h[11] - h[12] + h[13] - h[14];
+            gather(2, h);
+            dv += R(2) * (h[3] - h[4]) + h[7] - h[8] + h[10] - h[9] + h[15] - h[16] + h[17] - h[18];
+            gather(3, h);
+            dv += R(2) * (h[5] - h[6]) + h[11] - h[12] + h[14] - h[13] + h[15] - h[16] + h[18] - h[17];
+            a -= dv * W<R>::inv12;
+#pragma unroll
+            for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, o, lf[i] + a);
         }
     }
 }
@@ -491,8 +557,18 @@ struct Adj {
         else
             hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
                                g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
-        hipLaunchKernelGGL(k_adj_pass2<R>, grid_for(g.n), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
-                           (const R *)fields, lam_out, remap);
+        {
+            constexpr int TY = 4;
+            const int nbx = (g.nx + 63) / 64, nby = (g.ny + TY - 1) / TY;
+            int zchunk = 16;
+            while ((int64_t)nbx * nby * ((g.nz + zchunk - 1) / zchunk) < 2048 && zchunk > 4)
+                zchunk /= 2;
+            const int64_t nblk = (int64_t)nbx * nby * ((g.nz + zchunk - 1) / zchunk);
+            if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_adj_pass2<R, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
+                               g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk, remap,
+                               (const R *)fields, lam_out);
+        }
         if (grads)
             hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();
