@@ -263,8 +263,7 @@ int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
  * cell per lane) or registers (two cells per lane) instead of loading them
  * twice, and bits
  * 8..15 a number of KiB (<= 128) of unused LDS per block that limits how
- * many blocks share a CU.  Bits 1-2 only apply to the FP32-compute
- * precisions.  No reference counterpart; a
+ * many blocks share a CU.  Of bits 1-16, FP64 contexts use 1 and 32 only.  No reference counterpart; a
  * tuning knob of this implementation. */
 int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);
 

@@ -127,7 +127,8 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     // two cells per lane (32-bit words: twice the bytes per load) where rows
     // are a multiple of 128 cells, else from 6 waves per SIMD; both from the
     // stash of f_5 / f_6 (two fewer global loads per pulled cell, 1-2%
-    // faster; the FP32 kernel, bound by its registers, runs 2% slower with it)
+    // faster, FP64 too; the FP32 kernel, bound by its registers, runs 2%
+    // slower with it)
     const bool wide = nx % 128 == 0;
     const int stash = kFusedNtStore | kFusedZStash;
     if (precision == ZSC_F32)
@@ -137,7 +138,7 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     else if (precision == ZSC_FP16)
         c->cfg = wide ? FusedCfg{4, 24, stash | kFusedWide} : FusedCfg{8, 16, stash};
     else
-        c->cfg = {4, 16, kFusedNtStore};
+        c->cfg = {4, 16, stash};
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) {
         if (stream) {

@@ -1424,7 +1424,10 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
                                   void *phi)
 {
     if constexpr (sizeof(S) == 8) {
-        // FP64: streaming-store hint only (the other bits tune FP32 compute)
+        // FP64: streaming-store hint and f_5 / f_6 stash only (the other bits
+        // tune FP32 compute)
+        if ((cfg.flags & kFusedZStash) && (cfg.flags & kNtStore))
+            return launch_fused_v<S, TY, kNtStore | kZStash, false>(g, c, s, cfg, Bin, Bout, phi);
         if (cfg.flags & kNtStore) return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);
         return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
     } else {
