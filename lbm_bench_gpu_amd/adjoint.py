@@ -239,6 +239,9 @@ class AdjointRunner:
         return states, phis
 
     def _backward(self, lam, scale):
+        if self.tape is None and 0 not in self.checkpoints:
+            raise RuntimeError("backward needs the states of a forward pass run with keep=True "
+                               "(the last forward pass kept none)")
         h = self.ctx._h
         gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
         loss_slot = float(self.grads[14].item())
@@ -296,7 +299,9 @@ class AdjointRunner:
 
 class ZSCLoss(torch.autograd.Function):
     """L(theta) = sum (phi_N - phi_tgt)^2 of a ZSC run; theta is a float64
-    tensor (kappa, beta, tau_f, tau_g, gravity)."""
+    tensor (kappa, beta, tau_f, tau_g, gravity).  The runner holds the states
+    of its last forward pass, so one loss per runner may await its backward
+    pass at a time."""
 
     @staticmethod
     def forward(ctx, theta, phi0, phi_tgt, runner: AdjointRunner, nsteps: int):

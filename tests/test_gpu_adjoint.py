@@ -224,3 +224,41 @@ def test_deferred_coefficient_sums_equal_immediate(need_gpu, precision):
     print("deferred", d[:14].tolist(), "\nimmediate", i[:14].tolist(), flush=True)
     # the same FP64 sums in another order
     assert torch.allclose(d, i, rtol=1e-9, atol=1e-12 * float(i.abs().max()))
+
+
+def test_backward_needs_kept_states(need_gpu):
+    """A forward pass with keep=False (as loss_only runs) keeps no states;
+    a backward pass after it raises instead of summing over nothing."""
+    import torch
+    dims = (12, 10, 8)
+    p, phi0 = small_problem(dims, seed=6)
+    run = adjoint.AdjointRunner(dims, p, precision="f64", checkpoint_every=4)
+    theta = adjoint.theta_of(p)
+    a = run.forward(theta, torch.from_numpy(phi0), 5)
+    L, lam = run.loss(a, torch.zeros(run.n, dtype=torch.float64))
+    run.loss_only(theta, torch.from_numpy(phi0), torch.zeros(run.n, dtype=torch.float64), 5)
+    with pytest.raises(RuntimeError):
+        run.backward(lam)
+
+
+def test_autograd_function_gives_the_runner_gradients(need_gpu):
+    """zsc_loss (the torch.autograd.Function) through torch autograd: the
+    loss and theta.grad equal AdjointRunner.loss_and_grad's, and an upstream
+    gradient scales them (here 3 L)."""
+    import torch
+    dims = (12, 10, 8)
+    p, phi0 = small_problem(dims, seed=7)
+    theta0 = adjoint.theta_of(p)
+    tt = [theta0[0] * 1.2, theta0[1] * 0.9, theta0[2], theta0[3], theta0[4] * 1.5]
+    run = adjoint.AdjointRunner(dims, p, precision="f64", checkpoint_every=3)
+    phi0_t = torch.from_numpy(phi0)
+    tgt = run.forward(tt, phi0_t, 7, keep=False)[:7].sum(0).clone()
+    L_ref, g_ref = run.loss_and_grad(theta0, phi0_t, tgt, 7)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    L = adjoint.zsc_loss(theta, phi0_t, tgt, run, 7)
+    (3.0 * L).backward()
+    Lv = float(L.detach())
+    print("autograd", Lv, theta.grad.tolist(), "\nrunner  ", L_ref, g_ref, flush=True)
+    assert Lv == L_ref
+    for a, r in zip(theta.grad.tolist(), g_ref):
+        assert a == pytest.approx(3.0 * r, rel=1e-14, abs=0.0)
