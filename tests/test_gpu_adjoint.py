@@ -157,9 +157,10 @@ def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     central differences of the FP32 loss itself (h = 1e-2 .. 3e-3, where
     FP32 noise allows) kappa, beta and tau_f agree to 1e-4 .. 2.5e-6, so
     their distance from FP64 is the FP32 trajectory's; tau_g's derivative,
-    1.13 against O(1e4 .. 1e5) for the others, keeps ~1.5% of FP32 adjoint
-    rounding (FP64 coefficient sums remove only a fifth of it).  The bounds
-    are those measurements with room; the gravity derivative is 0."""
+    1.13 against O(1e4 .. 1e5) for the others, sums lambda (g - g_eq), whose
+    non-equilibrium part FP32 states (g stored absolutely) carry to ~1e-3
+    per cell: ~1.5% off.  The bounds are those measurements with room; the
+    gravity derivative is 0."""
     grads, fd = _workload_fd(precision, False)
     errs = {name: abs(a - r) / abs(r) for name, a, r in list(zip(adjoint.THETA, grads, fd))[:4]}
     print(precision, "relative error vs FD", errs, flush=True)
