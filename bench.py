@@ -458,7 +458,7 @@ def cpu_side_by_side():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "lbm_bench_gpu_amd runs the step on one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import torch
