@@ -2,7 +2,7 @@
 // reference's main() (src/main.cu:83-667) over the C-ABI, on one GPU.
 //
 //   zsc_run [-d DIR] [-p f64|f32|bf16|fp16] [-v ascii|binary|none]
-//           [-r RESTART_IN] [-s RESTART_OUT]
+//           [-r RESTART_IN] [-s RESTART_OUT] [-t TRIALS]
 //
 // Reads DIR/properties.in and DIR/discrete.in (default: the current
 // directory), runs relaxStep steps with gravity off and maxStep-1 steps with
@@ -12,7 +12,10 @@
 // data_00_NNNNNNN.vtk), and a runlog.out with the sections of
 // src/logsave.c:181-281, including the Laplace-law check after relaxation
 // (src/logsave.c:155-162).  -r resumes the evolution phase from a restart
-// file (no relaxation); -s writes one at the end.  Differences from the
+// file (no relaxation); -s writes one at the end.  -t keeps the fastest of
+// TRIALS lattice allocations (zsc_place_lattices; default 4, 0 for one
+// allocation): the step's time depends on the physical pages behind the
+// lattices.  Differences from the
 // reference: the statistics are reduced on the GPU in FP64 from the current
 // post-stream state (the reference uses moments one step old).
 #include <hip/hip_runtime.h>
@@ -99,6 +102,7 @@ int main(int argc, char **argv)
     std::string dir = ".", restart_in, restart_out;
     int prec = ZSC_F32;
     int vmode = 0;  // 0 ASCII (as the reference), 1 binary, -1 none
+    int trials = 4;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "-d") && i + 1 < argc) {
             dir = argv[++i];
@@ -106,6 +110,12 @@ int main(int argc, char **argv)
             restart_in = argv[++i];
         } else if (!strcmp(argv[i], "-s") && i + 1 < argc) {
             restart_out = argv[++i];
+        } else if (!strcmp(argv[i], "-t") && i + 1 < argc) {
+            trials = atoi(argv[++i]);
+            if (trials < 0 || trials > 16) {
+                fprintf(stderr, "zsc_run: -t takes 0..16 trials\n");
+                return 2;
+            }
         } else if (!strcmp(argv[i], "-v") && i + 1 < argc) {
             const char *v = argv[++i];
             vmode = !strcmp(v, "ascii") ? 0 : !strcmp(v, "binary") ? 1 : !strcmp(v, "none") ? -1 : -2;
@@ -123,7 +133,7 @@ int main(int argc, char **argv)
             }
         } else {
             fprintf(stderr, "usage: zsc_run [-d DIR] [-p f64|f32|bf16|fp16] [-v ascii|binary|none] "
-                            "[-r RESTART_IN] [-s RESTART_OUT]\n");
+                            "[-r RESTART_IN] [-s RESTART_OUT] [-t TRIALS]\n");
             return 2;
         }
     }
@@ -148,6 +158,13 @@ int main(int argc, char **argv)
     zsc_ctx *ctx = nullptr;
     CK(zsc_create(in.xmax, in.ymax, in.zmax, prec, nullptr, &ctx));
     CK(zsc_set_params(ctx, &p));
+    if (trials > 0) {
+        double ms[16];
+        CK(zsc_place_lattices(ctx, trials, ms));
+        printf("Lattice placement: %d trials, ms per step", trials);
+        for (int t = 0; t < trials; ++t) printf(" %.4f", ms[t]);
+        printf("\n\n");
+    }
     hipEvent_t e0, e1, e2, e3;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
