@@ -61,3 +61,14 @@ def test_library_errors_without_a_context():
     assert L.zsc_create(0, 4, 4, 0, None, ctypes.byref(ctypes.c_void_p())) == _lib.ZSC_ERR_ARG
     assert L.zsc_create(4, 4, 4, 9, None, ctypes.byref(ctypes.c_void_p())) == _lib.ZSC_ERR_ARG
     assert L.zsc_bytes_per_cell(7) == 0.0
+
+
+def test_makefile_tracks_every_header():
+    """Every header of csrc/ is a dependency of the objects (a header left out
+    of HDRS leaves a stale library after an edit to it)."""
+    csrc = os.path.join(ROOT, "lbm_bench_gpu_amd", "csrc")
+    mk = open(os.path.join(csrc, "Makefile")).read()
+    hdrs = re.search(r"^HDRS\s*:=\s*(.*)$", mk, re.M).group(1).split()
+    for h in sorted(f for f in os.listdir(csrc) if f.endswith(".h")):
+        assert h in hdrs, h
+    assert "../../include/zsc.h" in hdrs
