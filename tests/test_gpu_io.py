@@ -277,3 +277,36 @@ def test_driver_run_outputs_and_restart(need_gpu, tmp_path):
     key = b"SCALARS Phi float\nLOOKUP_TABLE default\n"
     got = np.frombuffer(raw, dtype=">f4", count=n ** 3, offset=raw.index(key) + len(key))
     assert np.array_equal(got, st.moments()[0].ravel().astype(np.float32))
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-4), ("bf16", 2e-3), ("fp16", 2e-4)])
+def test_driver_reduced_precision_with_placement_trials(need_gpu, tmp_path, prec, tol):
+    """zsc_run at FP32 / bf16 / fp16 storage with lattice placement trials
+    (-t 3) on a 128 x 24 x 24 box (rows of 128 cells: the two-cell kernel for
+    16-bit storage): the run completes with its output files and its final
+    phi stays within the stated bound of the FP64 oracle's (10 relaxation and
+    20 evolution steps; the VTK text holds 6 decimals)."""
+    nx, n = 128, 24
+    d = tmp_path / prec
+    d.mkdir()
+    props = PROPS.format(max_step=21, n=n).splitlines()
+    # the box: xmax ymax zmax on their own line of properties.in
+    text = "\n".join(props).replace(f"{n}\t{n}\t{n}", f"{nx}\t{n}\t{n}", 1)
+    assert f"{nx}\t{n}\t{n}" in text
+    (d / "properties.in").write_text(text + "\n")
+    (d / "discrete.in").write_text("nBubbles\n1\n\nxc\tyc\tzc\tRc\n64\t12\t12\t6\n")
+    r = _driver(["-d", str(d), "-p", prec, "-t", "3", "-v", "ascii"], str(d))
+    assert "Lattice placement: 3 trials" in r.stdout
+    for f in ["data_00_0000021.vtk", "stats.out", "runlog.out"]:
+        assert f in os.listdir(d), f
+    inp = zsc.read_inputs(str(d / "properties.in"))
+    b = zsc.read_bubbles(str(d / "discrete.in"))
+    p = zsc.derive_params(inp, float(b[0, 3]))
+    phi0 = O.init_phi((n, n, nx), b, p.phi_star, p.width)
+    st = O.State(phi0.shape, oracle_coef(p)).init(phi0)
+    st.run(10, grav_on=False)
+    st.run(20, grav_on=True)
+    got = _vtk_phi(d / "data_00_0000021.vtk", nx * n * n)
+    err = float(np.abs(got - st.moments()[0].ravel()).max())
+    print(prec, "zsc_run max|dphi| vs oracle", err, flush=True)
+    assert err <= tol
