@@ -73,12 +73,14 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags, trials=0):
     from lbm_bench_gpu_amd import zsc
     nx, ny, nz = w.dims
     ctx = zsc.Context(nx, ny, nz, precision=prec)
-    placement = None
+    placement, placement_s = None, 0.0
     try:
         configure(ctx, ty, zchunk, flags)
         ctx.set_params(params)
         if trials > 0:
+            tp = time.perf_counter()
             placement = [round(m, 4) for m in ctx.place_lattices(trials)]
+            placement_s = time.perf_counter() - tp
         ctx.state_init_bubbles(w.bubbles)
         ctx.step(warmup + 1)  # the first step also runs the A-lattice prologue
         ctx.sync()
@@ -104,6 +106,7 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags, trials=0):
         "sum_phi": st["sum_phi"],
         "finite": bool(np.isfinite(st["sum_phi"])),
         "placement_ms": placement,
+        "placement_s": round(placement_s, 3),
     }
 
 
@@ -532,6 +535,7 @@ def main():
         "roofline": main_r["roofline"],
         "kernel_ms": round(main_r["kernel_ms"], 4),
         "placement_ms": main_r["placement_ms"],
+        "placement_s": main_r["placement_s"],
         "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                          "speedup_vs_fp32": round(v["mlups"] / main_r["mlups"], 3),
                          "roofline": v["roofline"], "placement_ms": v["placement_ms"]}

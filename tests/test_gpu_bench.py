@@ -33,6 +33,6 @@ def test_bench_json_line(need_gpu):
     assert 0 < rf["frac"] < 1 and abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("port", "reference")
-    assert len(d["placement_ms"]) == 2 and min(d["placement_ms"]) > 0
+    assert len(d["placement_ms"]) == 2 and min(d["placement_ms"]) > 0 and d["placement_s"] > 0
     v = d["variants"]["bf16"]
     assert v["mlups"] > 0 and 0 < v["roofline"]["frac"] < 1
