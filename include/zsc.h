@@ -259,9 +259,9 @@ int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
  * FP32 storage, all 19 for 16-bit storage), 8 no
  * XCD-contiguous tile order, 16 (FP32-compute precisions, ty 4 or 8, nx a
  * multiple of 128; ignored otherwise) two cells per lane, 32 (with 1 and
- * without 2) keep f_5 / f_6, which consecutive planes share, in LDS (one
- * cell per lane) or registers (two cells per lane) instead of loading them
- * twice, and bits
+ * without 2; ignored for ty 1) keep f_5 / f_6, which consecutive planes
+ * share, in LDS (one cell per lane) or registers (two cells per lane)
+ * instead of loading them twice, and bits
  * 8..15 a number of KiB (<= 128) of unused LDS per block that limits how
  * many blocks share a CU.  Of bits 1-16, FP64 contexts use 1 and 32 only.  No reference counterpart; a
  * tuning knob of this implementation. */

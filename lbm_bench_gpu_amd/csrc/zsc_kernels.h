@@ -1425,9 +1425,10 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
 {
     if constexpr (sizeof(S) == 8) {
         // FP64: streaming-store hint and f_5 / f_6 stash only (the other bits
-        // tune FP32 compute)
-        if ((cfg.flags & kFusedZStash) && (cfg.flags & kNtStore))
-            return launch_fused_v<S, TY, kNtStore | kZStash, false>(g, c, s, cfg, Bin, Bout, phi);
+        // tune FP32 compute); one-row tiles ignore the stash
+        if constexpr (TY > 1)
+            if ((cfg.flags & kFusedZStash) && (cfg.flags & kNtStore))
+                return launch_fused_v<S, TY, kNtStore | kZStash, false>(g, c, s, cfg, Bin, Bout, phi);
         if (cfg.flags & kNtStore) return launch_fused_v<S, TY, 1, false>(g, c, s, cfg, Bin, Bout, phi);
         return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
     } else {
@@ -1442,9 +1443,11 @@ static hipError_t launch_fused_ty(const Geo &g, const Coef<typename Traits<S>::R
         }
         const int fl = cfg.flags & (kNtStore | kNtLoad);
         const bool pf = (cfg.flags & kFusedPrefetch) != 0;
-        if ((cfg.flags & kFusedZStash) && fl == kNtStore) {
-            if (pf) return launch_fused_v<S, TY, kNtStore | kZStash, true>(g, c, s, cfg, Bin, Bout, phi);
-            return launch_fused_v<S, TY, kNtStore | kZStash, false>(g, c, s, cfg, Bin, Bout, phi);
+        if constexpr (TY > 1) {  // one-row tiles ignore the stash
+            if ((cfg.flags & kFusedZStash) && fl == kNtStore) {
+                if (pf) return launch_fused_v<S, TY, kNtStore | kZStash, true>(g, c, s, cfg, Bin, Bout, phi);
+                return launch_fused_v<S, TY, kNtStore | kZStash, false>(g, c, s, cfg, Bin, Bout, phi);
+            }
         }
         switch (fl + (pf ? 4 : 0)) {
         case 0: return launch_fused_v<S, TY, 0, false>(g, c, s, cfg, Bin, Bout, phi);
