@@ -13,11 +13,21 @@ Pinning of each function (the checks live in ``tests/test_oracle.py``):
   zo_derive           ZSC identities sigma = (4/3) sqrt(2 kappa beta) phi*^3 and
                       W = sqrt(2 kappa / beta) / phi*, plus the D3Q19 weights.
   zo_init_phi         tanh profile values, last-bubble-wins, 1-based z.
-  zo_init_f/_g        sum f = phi, sum g = rho, zero momentum at rest.
+  zo_init_f/_g        the moments of the ZSC equilibria at u = 0 on a random
+                      phi (mu != 0): sum f = phi, sum c f = 0,
+                      sum c_a c_b f = Gamma mu d_ab; sum g = rho, sum c g = 0,
+                      sum c_a c_b g = (rho/3 + phi mu) d_ab; f_1 = Gamma mu / 2
+                      recovers the Laplacian's Fourier symbol.
   zo_update_*         brute-force NumPy sums on random states.
   zo_collision_f/_g   stencil symbols on Fourier modes, sum f_B = phi,
                       sum g_B = rho, the momentum closed form of the reference's
-                      forcing, equilibrium fixed point.
+                      forcing; the first and second moments of the f collision
+                      (C_f = phi invEta2 / tau_phi, A_f = Gamma mu / (2 tau_phi));
+                      the advection speed eta / (1 - eta) u of a phi mode in a
+                      uniform flow; fixed points at phi = +-phi* and at uniform
+                      phi with mu != 0 (the phi*mu terms of collision_g).
+                      Each of these checks fails on a planted mistake in the
+                      term it pins (zo_set_mutation).
   zo_stream_f/_g      permutation/conservation, delta-input smoothing closed form.
   zo_run              an independent NumPy transcription that keeps the
                       reference's ghost planes, neighbour tables and pack/unpack
@@ -79,12 +89,44 @@ def lib():
         L.zo_run.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef), _I, _I, _P, _P, _P]
         L.zo_stats.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef)] + [_P] * 6
         L.zo_pressure.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef)] + [_P] * 3
+        L.zo_set_mutation.argtypes = [_I]
         for f in ("zo_derive", "zo_init_phi", "zo_init_f", "zo_init_g", "zo_update_phi",
                   "zo_update_rho", "zo_update_velocity", "zo_collision_f", "zo_collision_g",
-                  "zo_stream_f", "zo_stream_g", "zo_run", "zo_stats", "zo_pressure"):
+                  "zo_stream_f", "zo_stream_g", "zo_run", "zo_stats", "zo_pressure",
+                  "zo_set_mutation"):
             getattr(L, f).restype = None
         _lib = L
     return _lib
+
+
+# Planted mistakes (zsc_oracle.h): each changes one term of one function.
+MUT_NONE = 0
+MUT_AF_NO_TAU = 1
+MUT_CF_NO_TAU = 2
+MUT_CF_HALF = 3
+MUT_F0_ONE_AF = 4
+MUT_INITG_REST_3PHIMU = 5
+MUT_INITG_AXIS_PHIMU = 6
+MUT_INITF_AF_NO_HALF = 7
+MUT_CG_REST_3PHIMU = 8
+MUT_CG_DIAG_6PHIMU = 9
+
+
+class mutation:
+    """Context manager that plants one mistake in the oracle for the duration
+    of a block (sensitivity tests of the pins), then restores the reference's
+    arithmetic."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        lib().zo_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().zo_set_mutation(MUT_NONE)
+        return False
 
 
 def _p(a: np.ndarray):

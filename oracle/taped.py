@@ -61,7 +61,7 @@ def step(f, g, c, grav_on=True):
         a = (i - 1) // 2
         s = 1.0 if i % 2 == 1 else -1.0
         fb.append(t * f[i] + af + s * cf * v[a])
-    # D3Q19 collision with the reference's forcing (src/collision_g.cu:228-313)
+    # D3Q19 collision with the reference's forcing (src/collision_g.cu:105-194)
     ut = [3.0 * v[a] for a in range(3)]
     uf = v[0] * F[0] + v[1] * F[1] + v[2] * F[2]
     usq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2]

@@ -38,6 +38,14 @@ static const int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1
 #define WC1 (1.0 / 6.0)
 #define WC2 (1.0 / 12.0)
 
+/* Planted mistakes for the sensitivity tests of the oracle's pins: a value
+ * other than ZO_MUT_NONE changes one term of one function the way a
+ * plausible slip would (zsc_oracle.h lists them).  ZO_MUT_NONE, the default,
+ * is the reference's arithmetic, evaluated exactly as without the switch. */
+static int g_mut = ZO_MUT_NONE;
+
+void zo_set_mutation(int m) { g_mut = m; }
+
 static inline int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 static inline size_t cell(int nx, int ny, int x, int y, int z)
@@ -155,7 +163,7 @@ void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const double *phi,
                 double phin = p[0];
                 double lap = lap19(p);
                 double mu = c->alpha4 * phin * (phin * phin - c->phi_star_sq) - c->kappa * lap;
-                double af = 0.5 * c->gamma * mu;
+                double af = g_mut == ZO_MUT_INITF_AF_NO_HALF ? c->gamma * mu : 0.5 * c->gamma * mu;
                 f[idx] = phin - 3.0 * c->gamma * mu;
                 for (int i = 1; i < 7; ++i) f[i * n + idx] = af;
             }
@@ -177,9 +185,11 @@ void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const double *phi,
                 double rhon = rho[idx];
                 double lap = lap19(p);
                 double mu = c->alpha4 * phin * (phin * phin - c->phi_star_sq) - c->kappa * lap;
-                double ag1 = W1 * (3.0 * phin * mu + rhon);
+                double ag1 = g_mut == ZO_MUT_INITG_AXIS_PHIMU ? W1 * (phin * mu + rhon)
+                                                              : W1 * (3.0 * phin * mu + rhon);
                 double ag2 = W2 * (3.0 * phin * mu + rhon);
-                g[idx] = W0 * (rhon - 6.0 * phin * mu);
+                g[idx] = g_mut == ZO_MUT_INITG_REST_3PHIMU ? W0 * (rhon - 3.0 * phin * mu)
+                                                           : W0 * (rhon - 6.0 * phin * mu);
                 for (int i = 1; i < 7; ++i) g[i * n + idx] = ag1;
                 for (int i = 7; i < 19; ++i) g[i * n + idx] = ag2;
             }
@@ -264,7 +274,11 @@ void zo_collision_f(int nx, int ny, int nz, const zo_coef *c, int grav_on,
                 double af = 0.5 * c->gamma * c->inv_tau_phi * mu;
                 double cf = c->inv_tau_phi * c->inv_eta2 * phin;
                 double t = c->inv_tau_phi_one;
-                fb[idx] = t * fa[idx] - 6.0 * af + c->inv_tau_phi * phin;
+                if (g_mut == ZO_MUT_AF_NO_TAU) af = 0.5 * c->gamma * mu;
+                if (g_mut == ZO_MUT_CF_NO_TAU) cf = c->inv_eta2 * phin;
+                if (g_mut == ZO_MUT_CF_HALF) cf = c->inv_tau_phi * 0.5 * phin;
+                fb[idx] = g_mut == ZO_MUT_F0_ONE_AF ? t * fa[idx] - af + c->inv_tau_phi * phin
+                                                    : t * fa[idx] - 6.0 * af + c->inv_tau_phi * phin;
                 fb[n + idx] = t * fa[n + idx] + af + cf * uxn;
                 fb[2 * n + idx] = t * fa[2 * n + idx] + af - cf * uxn;
                 fb[3 * n + idx] = t * fa[3 * n + idx] + af + cf * uyn;
@@ -307,7 +321,7 @@ void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
                 double eq1, eq2;
 
                 GB(0) = t * GA(0) +
-                        c->k0 * (rhon - 6.0 * phin * mu -
+                        c->k0 * (rhon - (g_mut == ZO_MUT_CG_REST_3PHIMU ? 3.0 : 6.0) * phin * mu -
                                  rhon * (uxn * uxn + uyn * uyn + uzn * uzn) / 6.0) -
                         c->kc0 * uf;
 
@@ -328,7 +342,7 @@ void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
                 GB(5) = t * GA(5) + eq1 + eq2;
                 GB(6) = t * GA(6) + eq1 - eq2;
 
-                temp = K2 * (3.0 * phin * mu +
+                temp = K2 * ((g_mut == ZO_MUT_CG_DIAG_6PHIMU ? 6.0 : 3.0) * phin * mu +
                              rhon * (1.0 - (uxn * uxn + uyn * uyn + uzn * uzn) / 6.0));
 
 #define DIAG(ip, im, s, S)                                                   \

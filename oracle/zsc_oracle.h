@@ -32,6 +32,24 @@ typedef struct zo_coef {
     double width, sigma;
 } zo_coef;
 
+/* Planted mistakes, for tests that show a pin is sensitive: each value
+ * changes one term the way a plausible slip would.  ZO_MUT_NONE (the default)
+ * leaves every function as the reference computes it.  The setting is
+ * process-wide; tests set it and reset it to ZO_MUT_NONE. */
+enum {
+    ZO_MUT_NONE = 0,
+    ZO_MUT_AF_NO_TAU = 1,          /* collision_f: A_f = Gamma mu / 2, 1/tau_phi dropped */
+    ZO_MUT_CF_NO_TAU = 2,          /* collision_f: C_f = phi invEta2, 1/tau_phi dropped */
+    ZO_MUT_CF_HALF = 3,            /* collision_f: C_f = phi / (2 tau_phi), invEta2 -> 1/2 */
+    ZO_MUT_F0_ONE_AF = 4,          /* collision_f: f_B0 subtracts A_f once, not 6 A_f */
+    ZO_MUT_INITG_REST_3PHIMU = 5,  /* init_g: g_0 = W0 (rho - 3 phi mu) */
+    ZO_MUT_INITG_AXIS_PHIMU = 6,   /* init_g: g_1..6 = W1 (phi mu + rho) */
+    ZO_MUT_INITF_AF_NO_HALF = 7,   /* init_f: f_1..6 = Gamma mu */
+    ZO_MUT_CG_REST_3PHIMU = 8,     /* collision_g: K0 (rho - 3 phi mu - ...) */
+    ZO_MUT_CG_DIAG_6PHIMU = 9      /* collision_g: diagonal K2 (6 phi mu + ...) */
+};
+void zo_set_mutation(int m);
+
 /* Parameter derivation of src/paramread.c:141-170 in FP64.  r0 is the radius
  * of bubble 0 (src/paramread.c:161).  phi_star_override > 0 replaces
  * 0.5*(rho_h-rho_l); gravity_override >= 0 replaces the Eo-based gravity. */

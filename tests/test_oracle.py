@@ -410,3 +410,182 @@ def test_pressure_sums_of_stats():
     inner[:, 1:-1, 1:-1] = True
     assert s[7] == pytest.approx(p[inner & (phi >= 0)].sum(), rel=1e-13)
     assert s[8] == pytest.approx(p[inner & (phi < 0)].sum(), rel=1e-13)
+
+
+# ------------------------- phi*mu terms and the f-collision amplitudes --
+# Each pin below is evaluated once on the oracle as it is and once with a
+# planted mistake in the term it pins (oracle.mutation); the mistake must
+# make it fail, which shows the check can see that term.
+
+def mu_weights(c, phi):
+    """Chemical potential mu = 4 beta phi (phi^2 - phi*^2) - kappa lap(phi)
+    with the Laplacian in its weight form."""
+    return 4.0 * c.alpha * phi * (phi * phi - c.phi_star_sq) - c.kappa * lap_weights(phi)
+
+
+def init_moments(c, phi, rho):
+    """Moments of init_f / init_g: (sum f, sum c f, sum c c f), same for g."""
+    n = phi.shape
+    f = np.zeros((7,) + n)
+    g = np.zeros((19,) + n)
+    O.lib().zo_init_f(n[2], n[1], n[0], O.ctypes.byref(c), O._p(phi), O._p(f))
+    O.lib().zo_init_g(n[2], n[1], n[0], O.ctypes.byref(c), O._p(phi), O._p(rho), O._p(g))
+
+    def mom(d, q):
+        m0 = d.sum(0)
+        m1 = [np.tensordot(C[:q, a], d, axes=(0, 0)) for a in range(3)]
+        m2 = [[np.tensordot(C[:q, a] * C[:q, b], d, axes=(0, 0)) for b in range(3)] for a in range(3)]
+        return m0, m1, m2
+
+    return mom(f, 7), mom(g, 19)
+
+
+def init_moment_errors(c, phi, rho):
+    """Largest deviation of the initial equilibria from the moments of the
+    ZSC equilibria at u = 0 (Zheng, Shu & Chew, J. Comput. Phys. 218 (2006)):
+    order parameter  sum f = phi,  sum c f = 0,  sum c_a c_b f = Gamma mu d_ab;
+    momentum         sum g = rho,  sum c g = 0,  sum c_a c_b g = (rho/3 + phi mu) d_ab."""
+    mu = mu_weights(c, phi)
+    (f0, f1, f2), (g0, g1, g2) = init_moments(c, phi, rho)
+    ef = [np.abs(f0 - phi).max()] + [np.abs(m).max() for m in f1]
+    eg = [np.abs(g0 - rho).max() / rho.max()] + [np.abs(m).max() for m in g1]
+    for a in range(3):
+        for b in range(3):
+            ef.append(np.abs(f2[a][b] - (c.gamma * mu if a == b else 0.0)).max())
+            want = rho / 3 + phi * mu if a == b else 0.0
+            eg.append(np.abs(g2[a][b] - want).max())
+    return max(ef), max(eg)
+
+
+def test_init_equilibria_moments_with_nonzero_mu():
+    """Random phi (mu of order 1e-2) and random rho: every phi*mu term of
+    init_g (src/init_g.cu:77-91) and the Gamma mu amplitude of init_f
+    (src/init_f.cu:72-78) enters one of these moments."""
+    shape = (6, 7, 8)
+    rng = np.random.default_rng(21)
+    phi = rng.uniform(-1.2, 1.2, shape)
+    rho = rng.uniform(1.0, 10.0, shape)
+    c = O.derive(phys(gamma=0.7, sigma=0.05))
+    assert np.abs(mu_weights(c, phi)).max() > 1e-2
+    ef, eg = init_moment_errors(c, phi, rho)
+    assert ef < 1e-14 and eg < 1e-14
+    for m in (O.MUT_INITG_REST_3PHIMU, O.MUT_INITG_AXIS_PHIMU):
+        with O.mutation(m):
+            assert init_moment_errors(c, phi, rho)[1] > 1e-4, m
+    with O.mutation(O.MUT_INITF_AF_NO_HALF):
+        assert init_moment_errors(c, phi, rho)[0] > 1e-4
+
+
+def uniform_drift(phi0, steps=4, **kw):
+    """Largest change of f and g over `steps` steps from the initial
+    equilibria of a uniform phi = phi0 without gravity."""
+    c = O.derive(phys(gravity=0.0, **kw))
+    shape = (4, 5, 6)
+    st = O.State(shape, c).init(np.full(shape, phi0))
+    f0, g0 = st.f.copy(), st.g.copy()
+    st.run(steps, grav_on=False)
+    return float(np.abs(st.f - f0).max()), float(np.abs(st.g - g0).max())
+
+
+@pytest.mark.parametrize("phi0", [0.3, -0.6, 1.4])
+def test_uniform_state_with_nonzero_mu_is_a_fixed_point(phi0):
+    """A uniform phi away from 0 and +-phi* has mu = 4 beta phi (phi^2 - phi*^2)
+    != 0 but no gradient, hence no force and no flow: the state must not
+    change.  The collision leaves f_1..6 = Gamma mu / 2 (pinned by the Fourier
+    symbol test) unchanged only if A_f = Gamma mu / (2 tau_phi)
+    (src/collision_f.cu:105), f_0 only with 6 A_f, and g only if the phi*mu
+    terms of collision_g (src/collision_g.cu:113-194) equal init_g's."""
+    kw = dict(tau_phi=0.7, tau_rho=0.8, gamma=0.9, sigma=0.05)
+    ef, eg = uniform_drift(phi0, **kw)
+    assert ef < 1e-15 and eg < 1e-14
+    for m in (O.MUT_AF_NO_TAU, O.MUT_F0_ONE_AF):
+        with O.mutation(m):
+            assert uniform_drift(phi0, **kw)[0] > 1e-5, m
+    for m in (O.MUT_CG_REST_3PHIMU, O.MUT_CG_DIAG_6PHIMU):
+        with O.mutation(m):
+            assert uniform_drift(phi0, **kw)[1] > 1e-5, m
+
+
+def collision_f_moment_errors(c, st, grav_on=True):
+    """One f collision on a random state against the moment closed forms of
+    a BGK relaxation towards the ZSC equilibrium with forcing-shifted velocity
+    u' = u + F / (2 rho), F = mu grad(phi) (+ gravity on phi > 0 in z):
+      sum c_a f_B       = (1 - 1/tau) sum c_a f_A + phi u'_a / (tau (1 - eta))
+      sum c_a c_b f_B   = (1 - 1/tau) sum c_a c_b f_A + Gamma mu d_ab / tau
+    The 1/(1 - eta) = 2 invEta2 factor compensates the smoothing of the
+    streaming (src/paramread.c:147-148); its macroscopic effect is checked in
+    test_phi_advection_speed_closed_form."""
+    phi, rho, ux, uy, uz = st.moments()
+    fb, _ = O.collide(c, st.f, st.g, grav_on=grav_on)
+    mu = mu_weights(c, phi)
+    grad = grad_weights(phi)
+    F = [mu * gr for gr in grad]
+    if grav_on:
+        F[2] = F[2] + np.where(phi > 0, c.grav, 0.0)
+    tau = 1.0 / c.inv_tau_phi
+    e1, e2 = [], []
+    for a, u in enumerate((ux, uy, uz)):
+        up = u + F[a] / (2 * rho)
+        m_a = np.tensordot(C[:7, a], st.f, axes=(0, 0))
+        m_b = np.tensordot(C[:7, a], fb, axes=(0, 0))
+        e1.append(np.abs(m_b - ((1 - 1 / tau) * m_a + phi * up / (tau * (1 - c.eta)))).max())
+        for b in range(3):
+            w = C[:7, a] * C[:7, b]
+            s_a = np.tensordot(w, st.f, axes=(0, 0))
+            s_b = np.tensordot(w, fb, axes=(0, 0))
+            want = (1 - 1 / tau) * s_a + (c.gamma * mu / tau if a == b else 0.0)
+            e2.append(np.abs(s_b - want).max())
+    return max(e1), max(e2)
+
+
+def test_collision_f_moments_pin_af_and_cf():
+    """First moment fixes C_f = phi invEta2 / tau_phi, second moment fixes
+    A_f = Gamma mu / (2 tau_phi) (src/collision_f.cu:105-114), on a random
+    state with flow, interfacial force and gravity."""
+    c = O.derive(phys(tau_phi=0.7, gamma=0.8, sigma=0.05, gravity=1e-3))
+    st = random_state((6, 7, 8), 22, c)
+    st.g += 0.02 * np.random.default_rng(23).standard_normal(st.g.shape)  # flow
+    e1, e2 = collision_f_moment_errors(c, st)
+    assert e1 < 1e-14 and e2 < 1e-14
+    for m in (O.MUT_CF_NO_TAU, O.MUT_CF_HALF):
+        with O.mutation(m):
+            assert collision_f_moment_errors(c, st)[0] > 1e-5, m
+    with O.mutation(O.MUT_AF_NO_TAU):
+        assert collision_f_moment_errors(c, st)[1] > 1e-5
+
+
+def advection_speed(tau_phi, u0=0.01, nx=48, steps=240):
+    """Speed at which a small cosine mode of phi travels in a uniform flow u0
+    along x, fitted from the phase of its Fourier coefficient."""
+    c = O.derive(phys(tau_phi=tau_phi, gravity=0.0))
+    shape = (3, 3, nx)
+    k = 2 * np.pi / nx
+    phi = np.broadcast_to(0.02 * np.cos(k * np.arange(nx)), shape).copy()
+    st = O.State(shape, c).init(phi)
+    # momentum populations at the D3Q19 equilibrium of rho_mean moving at u0
+    cu = 3.0 * C[:, 0] * u0
+    geq = WQ * c.rho_mean * (1 + cu + 0.5 * cu * cu - 1.5 * u0 * u0)
+    st.g = np.ascontiguousarray(np.broadcast_to(geq[:, None, None, None], (19,) + shape))
+    phase = []
+    for _ in range(steps):
+        st.run(1, grav_on=False)
+        phase.append(np.angle(np.fft.rfft(st.f.sum(0).mean(axis=(0, 1)))[1]))
+    phase = np.unwrap(phase)[steps // 2:]
+    t = np.arange(steps // 2, steps)
+    return -np.polyfit(t, phase, 1)[0] / k, c
+
+
+@pytest.mark.parametrize("tau_phi", [0.7, 1.5])
+def test_phi_advection_speed_closed_form(tau_phi):
+    """Macroscopic pin of C_f.  The collision relaxes the first moment of f
+    to phi u / (1 - eta); the smoothed pull f_A,i(x) = eta f_B,i(x - c_i) +
+    (1 - eta) f_B,i(x) (src/stream_f.cu:52-59) moves only the eta part, so
+    the flux of phi is eta phi u / (1 - eta) and a small mode of phi travels
+    at v = eta / (1 - eta) u0 (5 u0 at tau_phi = 0.7, u0 at 1.5).  The
+    fitted speed includes the lattice dispersion of a mode of wavelength 48,
+    well under the 3% tolerance."""
+    v, c = advection_speed(tau_phi)
+    want = c.eta / (1 - c.eta) * 0.01
+    assert v == pytest.approx(want, rel=0.03)
+    with O.mutation(O.MUT_CF_NO_TAU):
+        assert abs(advection_speed(tau_phi)[0] / want - 1) > 0.2
