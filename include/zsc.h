@@ -232,6 +232,15 @@ typedef struct zsc_stats {
 /* Reduces the moments of the A state.  Returns ZSC_ERR_DIVERGED when a sum
  * is NaN or Inf. */
 int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
+/* The same diagnostics over reference-style device moment arrays (N
+ * elements each at the context's element type, as zsc_update_phi, _rho and
+ * _velocity leave them): the drop-in counterpart of src/stats.c:39-91 and
+ * the sums of src/logsave.c:63-143, which the reference forms on the host
+ * after copying phi and u back.  F64 and F32 contexts; needs parameters
+ * (the pressure sums).  The context allocates an 82 KB buffer for the
+ * partial sums on first use. */
+int zsc_moment_stats(zsc_ctx *ctx, const void *phi, const void *rho, const void *ux,
+                     const void *uy, const void *uz, zsc_stats *out);
 
 /* ---- output and restart ------------------------------------------------ */
 /* Pressure of src/vtksave.c:75-119 (periodic 19-point stencil of phi, at the
@@ -251,6 +260,18 @@ int zsc_vtk_save(zsc_ctx *ctx, const char *path, int binary);
  * another box or precision and ZSC_ERR_IO for a damaged file. */
 int zsc_state_save(zsc_ctx *ctx, const char *path, int64_t step);
 int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
+/* Same, with up to ZSC_RESTART_USER_MAX caller values stored beside the
+ * step counter (e.g. a driver's reference bubble volume, so that a resumed
+ * run reports the same phase conservation as one that did not stop).
+ * zsc_state_load_ex copies min(capacity, stored) values to user and the
+ * stored count to *n_user (may be NULL).  Files without caller values load
+ * with *n_user = 0.  Parameters in the file are validated as by
+ * zsc_set_params (ZSC_ERR_IO otherwise, before the state is touched). */
+#define ZSC_RESTART_USER_MAX 8
+int zsc_state_save_ex(zsc_ctx *ctx, const char *path, int64_t step, const double *user,
+                      int n_user);
+int zsc_state_load_ex(zsc_ctx *ctx, const char *path, int64_t *step, double *user,
+                      int capacity, int *n_user);
 
 /* Launch shape of the fused kernel: ty wavefront rows of 64 cells per block
  * (1, 2, 4 or 8), zchunk planes marched per block (0: the whole box) and
@@ -266,6 +287,9 @@ int zsc_state_load(zsc_ctx *ctx, const char *path, int64_t *step);
  * many blocks share a CU.  Of bits 1-16, FP64 contexts use 1 and 32 only.  No reference counterpart; a
  * tuning knob of this implementation. */
 int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);
+/* The launch shape in use (the per-precision default until set); any
+ * pointer may be NULL. */
+int zsc_get_fused_config(zsc_ctx *ctx, int *ty, int *zchunk, int *flags);
 
 /* ---- adjoint (no reference counterpart: the reference has no derivatives) */
 /* Hand-written reverse mode of the time step for F64 and F32 contexts.  All

@@ -66,6 +66,7 @@ SIGNATURES = {
     "zsc_set_params": (_I, [_VP, ctypes.POINTER(ZscParams)]),
     "zsc_set_gravity": (_I, [_VP, _I]),
     "zsc_set_fused_config": (_I, [_VP, _I, _I, _I]),
+    "zsc_get_fused_config": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "zsc_sync": (_I, [_VP]),
     "zsc_last_error": (ctypes.c_char_p, []),
     "zsc_init_f": (_I, [_VP, _VP, _PP7]),
@@ -88,10 +89,14 @@ SIGNATURES = {
     "zsc_state_export": (_I, [_VP, _PP7, _PP19]),
     "zsc_state_import": (_I, [_VP, _PP7, _PP19]),
     "zsc_state_stats": (_I, [_VP, ctypes.POINTER(ZscStats)]),
+    "zsc_moment_stats": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(ZscStats)]),
     "zsc_state_pressure": (_I, [_VP, _DP]),
     "zsc_vtk_save": (_I, [_VP, ctypes.c_char_p, _I]),
     "zsc_state_save": (_I, [_VP, ctypes.c_char_p, ctypes.c_int64]),
     "zsc_state_load": (_I, [_VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "zsc_state_save_ex": (_I, [_VP, ctypes.c_char_p, ctypes.c_int64, _DP, _I]),
+    "zsc_state_load_ex": (_I, [_VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), _DP, _I,
+                               ctypes.POINTER(_I)]),
     "zsc_bytes_per_cell": (_D, [_I]),
     "zsc_adj_scratch_bytes": (ctypes.c_size_t, [_VP]),
     "zsc_adj_init": (_I, [_VP, _VP, _VP]),
@@ -104,6 +109,8 @@ SIGNATURES = {
     "zsc_adj_stream": (_I, [_VP, _VP, _VP]),
     "zsc_adj_init_backward": (_I, [_VP, _VP, _VP, _VP, _VP]),
 }
+
+RESTART_USER_MAX = 8  # ZSC_RESTART_USER_MAX
 
 _lib = None
 

@@ -171,12 +171,24 @@ extern "C" int zsc_destroy(zsc_ctx *c)
     return ZSC_OK;
 }
 
+// The scalars a step divides by or relaxes with must be finite, and the
+// smoothing weight eta must lie in (0, 1) so that invEta2 = 1/(2(1 - eta))
+// exists (src/paramread.c:147-148).
+static bool params_valid(const zsc_params &p)
+{
+    if (!(p.eta > 0.0 && p.eta < 1.0) || !isfinite(p.inv_eta2)) return false;
+    const double *v = (const double *)&p;
+    for (size_t i = 0; i < sizeof(zsc_params) / sizeof(double); ++i)
+        if (!isfinite(v[i])) return false;
+    return true;
+}
+
 extern "C" int zsc_set_params(zsc_ctx *c, const zsc_params *p)
 {
     if (int r = need_ctx(c)) return r;
     if (!p) return fail(ZSC_ERR_ARG, "zsc_set_params: null params");
-    if (!(p->eta > 0.0 && p->eta < 1.0) || !isfinite(p->inv_eta2))
-        return fail(ZSC_ERR_ARG, "zsc_set_params: eta must lie in (0,1)");
+    if (!params_valid(*p))
+        return fail(ZSC_ERR_ARG, "zsc_set_params: eta must lie in (0,1) and every scalar be finite");
     c->params = *p;
     c->have_params = true;
     return ZSC_OK;
@@ -196,6 +208,15 @@ extern "C" int zsc_set_fused_config(zsc_ctx *c, int ty, int zchunk, int flags)
         (flags & 0xc0) != 0 || (flags >> 8) > 128 || (flags & 3) == kFusedNtLoad)
         return fail(ZSC_ERR_ARG, "zsc_set_fused_config: ty in {1,2,4,8}, zchunk >= 0, valid flags");
     c->cfg = {ty, zchunk, flags};
+    return ZSC_OK;
+}
+
+extern "C" int zsc_get_fused_config(zsc_ctx *c, int *ty, int *zchunk, int *flags)
+{
+    if (int r = need_ctx(c)) return r;
+    if (ty) *ty = c->cfg.ty;
+    if (zchunk) *zchunk = c->cfg.zchunk;
+    if (flags) *flags = c->cfg.flags;
     return ZSC_OK;
 }
 
@@ -345,13 +366,12 @@ static int ensure_alloc(zsc_ctx *c)
         return ZSC_OK;
     });
     const size_t wb = 5 * align256((size_t)c->geo.n * c->rsize);
-    void *a = nullptr, *b = nullptr, *w = nullptr, *s = nullptr;
+    void *a = nullptr, *b = nullptr, *w = nullptr;
     hipError_t e = hipMalloc(&a, lb);
     if (e == hipSuccess) e = hipMalloc(&b, lb);
     if (e == hipSuccess) e = hipMalloc(&w, wb);
-    if (e == hipSuccess) e = hipMalloc(&s, (kStatsBlocks + 1) * 10 * sizeof(double));
     if (e != hipSuccess) {
-        for (void *p : {a, b, w, s})
+        for (void *p : {a, b, w})
             if (p) (void)hipFree(p);
         (void)hipGetLastError();
         return fail(ZSC_ERR_ALLOC, "cannot allocate %.2f GB of lattices and work arrays",
@@ -361,7 +381,22 @@ static int ensure_alloc(zsc_ctx *c)
     c->lat[1] = b;
     c->lat_bytes = lb;
     c->work = w;
-    c->stats_buf = (double *)s;
+    return ZSC_OK;
+}
+
+// Results (10 doubles) and per-block partial sums of the statistics
+// reduction: a fixed grid of kStatsBlocks blocks strides over any box.
+static constexpr size_t kStatsScratch = (size_t)(kStatsBlocks + 1) * 10;
+
+static int ensure_stats_buf(zsc_ctx *c)
+{
+    if (c->stats_buf) return ZSC_OK;
+    if (int r = set_device(c)) return r;
+    if (hipMalloc((void **)&c->stats_buf, kStatsScratch * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        c->stats_buf = nullptr;
+        return fail(ZSC_ERR_ALLOC, "cannot allocate the statistics buffer");
+    }
     return ZSC_OK;
 }
 
@@ -685,23 +720,22 @@ extern "C" int zsc_state_moments(zsc_ctx *c, double *phi, double *rho, double *u
     });
 }
 
-extern "C" int zsc_state_stats(zsc_ctx *c, zsc_stats *out)
+// Reduces five moment arrays (element type: the compute type of storage S)
+// with the statistics kernels, using buf (kStatsScratch doubles: results,
+// then per-block partials), and copies the ten results to h.  Every partial
+// and result is written before it is read: buf needs no clearing.
+template <class S>
+static hipError_t reduce_stats(zsc_ctx *c, const void *phi, const void *rho, const void *ux,
+                               const void *uy, const void *uz, double *buf, double h[10])
 {
-    if (int r = need_ctx(c)) return r;
-    if (!out) return fail(ZSC_ERR_ARG, "zsc_state_stats: null output");
-    if (int r = moments_to_work(c)) return r;
-    double h[10];
-    int r = dispatch(c->prec, [&](auto tag) {
-        using S = ZSC_T;
-        double *partial = c->stats_buf + 10;
-        ZSC_HIP(Ops<S>::stats(c->geo, c->params, c->stream, c->w(0), c->w(1), c->w(2), c->w(3),
-                              c->w(4), partial, c->stats_buf),
-                "stats");
-        ZSC_HIP(hipMemcpyAsync(h, c->stats_buf, sizeof h, hipMemcpyDeviceToHost, c->stream), "stats");
-        ZSC_HIP(hipStreamSynchronize(c->stream), "stats");
-        return ZSC_OK;
-    });
-    if (r) return r;
+    hipError_t e = Ops<S>::stats(c->geo, c->params, c->stream, phi, rho, ux, uy, uz, buf + 10, buf);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, buf, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+
+static int stats_out(const double h[10], zsc_stats *out)
+{
     out->sum_phi = h[0];
     out->sum_rho = h[1];
     out->bubble_volume = h[2];
@@ -716,6 +750,53 @@ extern "C" int zsc_state_stats(zsc_ctx *c, zsc_stats *out)
         return fail(ZSC_ERR_DIVERGED, "non-finite conserved sum (sum phi %g, sum rho %g)", h[0], h[1]);
     return ZSC_OK;
 }
+
+extern "C" int zsc_state_stats(zsc_ctx *c, zsc_stats *out)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!out) return fail(ZSC_ERR_ARG, "zsc_state_stats: null output");
+    if (int r = moments_to_work(c)) return r;
+    if (int r = ensure_stats_buf(c)) return r;
+    double h[10];
+    int r = dispatch(c->prec, [&](auto tag) {
+        using S = ZSC_T;
+        ZSC_HIP(reduce_stats<S>(c, c->w(0), c->w(1), c->w(2), c->w(3), c->w(4), c->stats_buf, h),
+                "stats");
+        return ZSC_OK;
+    });
+    if (r) return r;
+    return stats_out(h, out);
+}
+
+static int moment_stats(zsc_ctx *c, const void *phi, const void *rho, const void *ux,
+                        const void *uy, const void *uz, zsc_stats *out, double *buf, const char *name)
+{
+    double h[10];
+    int r = dropin(c, true, name, [&](auto tag) {
+        using T = ZSC_T;
+        return reduce_stats<T>(c, phi, rho, ux, uy, uz, buf, h);
+    });
+    if (r) return r;
+    return stats_out(h, out);
+}
+
+static bool moments_set(const void *phi, const void *rho, const void *ux, const void *uy,
+                        const void *uz, const zsc_stats *out)
+{
+    return phi && rho && ux && uy && uz && out;
+}
+
+extern "C" int zsc_moment_stats(zsc_ctx *c, const void *phi, const void *rho, const void *ux,
+                                const void *uy, const void *uz, zsc_stats *out)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!moments_set(phi, rho, ux, uy, uz, out)) return fail(ZSC_ERR_ARG, "zsc_moment_stats: null pointer");
+    if (c->prec != ZSC_F64 && c->prec != ZSC_F32)
+        return fail(ZSC_ERR_PRECISION, "zsc_moment_stats: drop-in arrays need an F64 or F32 context");
+    if (int r = ensure_stats_buf(c)) return r;
+    return moment_stats(c, phi, rho, ux, uy, uz, out, c->stats_buf, "zsc_moment_stats");
+}
+
 
 template <bool Export>
 static int xport(zsc_ctx *c, void *const *f, void *const *g, const char *name)
@@ -892,7 +973,9 @@ extern "C" int zsc_vtk_save(zsc_ctx *c, const char *path, int binary)
 
 namespace {
 // Restart file header, host byte order, followed by the lattice bytes.
-struct RestartHeader {
+// Version 2 appends up to ZSC_RESTART_USER_MAX caller values; version 1
+// files (no caller values) still load.
+struct RestartHeaderV1 {
     char magic[8];
     int32_t version, precision;
     int32_t nx, ny, nz;
@@ -901,19 +984,38 @@ struct RestartHeader {
     uint64_t lattice_bytes;
     zsc_params params;
 };
+struct RestartHeader : RestartHeaderV1 {
+    int32_t n_user, pad2;
+    double user[ZSC_RESTART_USER_MAX];
+};
 const char kRestartMagic[8] = {'Z', 'S', 'C', 'S', 'T', 'A', 'T', 'E'};
 const int64_t kRestartChunk = (int64_t)64 << 20;
 }  // namespace
 
 extern "C" int zsc_state_save(zsc_ctx *c, const char *path, int64_t step)
 {
+    return zsc_state_save_ex(c, path, step, nullptr, 0);
+}
+
+extern "C" int zsc_state_load(zsc_ctx *c, const char *path, int64_t *step)
+{
+    return zsc_state_load_ex(c, path, step, nullptr, 0, nullptr);
+}
+
+extern "C" int zsc_state_save_ex(zsc_ctx *c, const char *path, int64_t step, const double *user,
+                                 int n_user)
+{
     if (int r = need_ctx(c)) return r;
     if (!path) return fail(ZSC_ERR_ARG, "zsc_state_save: null path");
+    if (n_user < 0 || n_user > ZSC_RESTART_USER_MAX || (n_user > 0 && !user))
+        return fail(ZSC_ERR_ARG, "zsc_state_save: 0..%d caller values", ZSC_RESTART_USER_MAX);
     if (c->state == STATE_NONE) return fail(ZSC_ERR_STATE, "zsc_state_save: no state");
     RestartHeader h;
     memset(&h, 0, sizeof h);
     memcpy(h.magic, kRestartMagic, 8);
-    h.version = 1;
+    h.version = 2;
+    h.n_user = n_user;
+    for (int i = 0; i < n_user; ++i) h.user[i] = user[i];
     h.precision = c->prec;
     h.nx = c->geo.nx;
     h.ny = c->geo.ny;
@@ -944,16 +1046,30 @@ extern "C" int zsc_state_save(zsc_ctx *c, const char *path, int64_t step)
     return r;
 }
 
-extern "C" int zsc_state_load(zsc_ctx *c, const char *path, int64_t *step)
+extern "C" int zsc_state_load_ex(zsc_ctx *c, const char *path, int64_t *step, double *user,
+                                 int capacity, int *n_user)
 {
     if (int r = need_ctx(c)) return r;
     if (!path) return fail(ZSC_ERR_ARG, "zsc_state_load: null path");
+    if (capacity < 0 || (capacity > 0 && !user)) return fail(ZSC_ERR_ARG, "zsc_state_load: bad user buffer");
     FILE *fp = fopen(path, "rb");
     if (!fp) return fail(ZSC_ERR_IO, "zsc_state_load: cannot open %s", path);
     FileCloser closer{fp};
     RestartHeader h;
-    if (fread(&h, sizeof h, 1, fp) != 1 || memcmp(h.magic, kRestartMagic, 8) != 0 || h.version != 1)
+    memset(&h, 0, sizeof h);
+    RestartHeaderV1 &h1 = h;
+    if (fread(&h1, sizeof h1, 1, fp) != 1 || memcmp(h.magic, kRestartMagic, 8) != 0 ||
+        (h.version != 1 && h.version != 2))
         return fail(ZSC_ERR_IO, "zsc_state_load: %s is not a libzsc restart file", path);
+    if (h.version == 2) {
+        const size_t tail = sizeof h - sizeof h1;
+        if (fread((char *)&h + sizeof h1, tail, 1, fp) != 1 || h.n_user < 0 ||
+            h.n_user > ZSC_RESTART_USER_MAX)
+            return fail(ZSC_ERR_IO, "zsc_state_load: %s has a damaged header", path);
+    }
+    if (!params_valid(h.params))
+        return fail(ZSC_ERR_IO, "zsc_state_load: %s holds invalid parameters (eta outside (0,1) "
+                                "or a non-finite scalar)", path);
     if (h.precision != c->prec || h.nx != c->geo.nx || h.ny != c->geo.ny || h.nz != c->geo.nz)
         return fail(ZSC_ERR_ARG,
                     "zsc_state_load: file holds a %dx%dx%d box at precision %d, context is "
@@ -990,6 +1106,8 @@ extern "C" int zsc_state_load(zsc_ctx *c, const char *path, int64_t *step)
     c->have_params = true;
     c->grav_on = h.grav_on;
     if (step) *step = h.step;
+    for (int i = 0; i < std::min(capacity, (int)h.n_user); ++i) user[i] = h.user[i];
+    if (n_user) *n_user = h.n_user;
     return ZSC_OK;
 }
 

@@ -1,6 +1,8 @@
 // zsc_host.cpp -- host-side inputs of libzsc: the reference's two input files,
 // the parameter derivation, a seeded bubble generator and the host initial
 // order parameter.  Plain C++ (no HIP calls).
+#include <errno.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -40,8 +42,9 @@ struct Tokens {
     {
         if (pos >= t.size()) return false;
         char *end = nullptr;
+        errno = 0;
         long x = strtol(t[pos].c_str(), &end, 10);
-        if (!end || *end) return false;
+        if (!end || *end || errno == ERANGE || x < INT_MIN || x > INT_MAX) return false;
         *v = (int)x;
         ++pos;
         return true;

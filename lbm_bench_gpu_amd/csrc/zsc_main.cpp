@@ -23,6 +23,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
 #include <string>
@@ -48,6 +49,21 @@ struct StatsFile {
 };
 
 static const double kInvPi = 0.31830988618379067154;
+
+// "HH:MM on YYYY-MM-DD", the stamp of src/logsave.c:194-199,272-277
+static std::string stamp()
+{
+    char b[32] = "";
+    time_t t = time(nullptr);
+    struct tm tmv;
+    if (localtime_r(&t, &tmv)) strftime(b, sizeof b, "%H:%M on %Y-%m-%d", &tmv);
+    return b;
+}
+
+// Values a restart file carries for this driver (zsc_state_save_ex): the
+// bubble volume at the start of evolution, which every later Vef divides by
+// (src/stats.c:66,84), and the relaxation results of src/logsave.c:155-167.
+enum { kUserInitVol, kUserRelaxVef, kUserPdif, kUserPerr, kUserRef, kUserUmax, kUserCount };
 
 static void stats_line(const StatsFile &sf, int step, const zsc_stats &s, double inv_vol)
 {
@@ -148,10 +164,13 @@ int main(int argc, char **argv)
     if (in.stat <= 0) in.stat = 1 << 30;
     if (in.save <= 0) in.save = 1 << 30;
 
+    const std::string started = stamp();
     printf("*** Multiphase Zheng-Shu-Chew LBM 3D Simulation\n");
     printf("*** MI355X HIP implementation (libzsc)\n\n");
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
+    memset(&prop, 0, sizeof prop);
+    const bool have_prop = hipGetDeviceProperties(&prop, 0) == hipSuccess;
+    if (have_prop)
         printf("*** Program running on : %s\n*** Total GPU Memory   : %.4f GB\n\n", prop.name,
                prop.totalGlobalMem / 1073741824.0);
 
@@ -175,9 +194,14 @@ int main(int argc, char **argv)
     int64_t resumed = -1;
     double laplace[4] = {0, 0, 0, 0};  // Pdif, Perr, Ref, Umax after relaxation
     double relax_vef = 0.0;
+    double user[kUserCount] = {0, 0, 0, 0, 0, 0};
+    int n_user = 0;
     if (!restart_in.empty()) {
-        CK(zsc_state_load(ctx, restart_in.c_str(), &resumed));
+        CK(zsc_state_load_ex(ctx, restart_in.c_str(), &resumed, user, kUserCount, &n_user));
         printf("Restarted from %s (step %lld).\n", restart_in.c_str(), (long long)resumed);
+        if (n_user < kUserCount)
+            fprintf(stderr, "zsc_run: %s carries no reference bubble volume; Vef is relative to "
+                            "the resumed state\n", restart_in.c_str());
     } else {
         CK(zsc_state_init_bubbles(ctx, bub.data(), nb));
     }
@@ -229,7 +253,18 @@ int main(int argc, char **argv)
         }
     }
     CK(zsc_state_stats(ctx, &s));
-    const double inv_init = s.bubble_volume > 0 ? 1.0 / s.bubble_volume : 0.0;
+    // the reference volume of every evolution Vef: this state's, or on a
+    // resumed run the one the original run started its evolution with
+    const double init_vol = resumed >= 0 && n_user >= kUserCount ? user[kUserInitVol] : s.bubble_volume;
+    const double inv_init = init_vol > 0 ? 1.0 / init_vol : 0.0;
+    if (resumed < 0) {
+        user[kUserInitVol] = init_vol;
+        user[kUserRelaxVef] = relax_vef;
+        for (int i = 0; i < 4; ++i) user[kUserPdif + i] = laplace[i];
+    } else if (n_user >= kUserCount) {
+        relax_vef = user[kUserRelaxVef];
+        for (int i = 0; i < 4; ++i) laplace[i] = user[kUserPdif + i];
+    }
     if (resumed < 0) {
         stats_line(evo, 0, s, inv_init);
         CK(vtk(ctx, dir, vmode, false, 0));
@@ -251,20 +286,40 @@ int main(int argc, char **argv)
     // final configuration, named with the loop counter's end value maxStep
     // as src/main.cu:628-634 does
     CK(vtk(ctx, dir, vmode, false, in.max_step));
-    if (!restart_out.empty()) CK(zsc_state_save(ctx, restart_out.c_str(), last));
+    if (!restart_out.empty())
+        CK(zsc_state_save_ex(ctx, restart_out.c_str(), last, user, kUserCount));
     const double cells = (double)in.xmax * in.ymax * in.zmax;
     const int nrun = last >= first ? last - first + 1 : 0;
     const double mlups = nrun > 0 && evo_ms > 0 ? cells * nrun / (evo_ms * 1e3) : 0.0;
+    // the reference's own figures (src/main.cu:624, src/logsave.c:262-263):
+    // the main-loop time, stats and VTK output included, divided by maxStep
+    // although maxStep-1 steps ran, over a grid that counts the two z ghost
+    // planes of src/main.cu:161-164
+    const double ref_cells = (double)in.xmax * in.ymax * (in.zmax + 2.0);
+    const double ref_step_ms = in.max_step > 0 ? t_main / in.max_step : 0.0;
+    const double ref_mlups = t_main > 0 ? in.max_step * ref_cells * 1e-6 / (t_main * 1e-3) : 0.0;
     printf("Setup time      : %6.3f ms\n", t_setup);
     printf("Relaxation time : %6.3f ms\n", t_relax);
     printf("Evolution time  : %6.3f ms\n", t_main);
-    printf("Time for lb step: %6.3f ms\n", nrun > 0 ? evo_ms / nrun : 0.0);
-    printf("MLUPS (step kernels only): %.1f\n", mlups);
+    printf("Time for lb step: %6.3f ms (reference formula: main loop / maxStep)\n", ref_step_ms);
+    printf("MLUPS           : %.1f (reference formula: maxStep * nx*ny*(nz+2) / main loop)\n", ref_mlups);
+    printf("Step kernels    : %6.3f ms per step, %.1f MLUPS (nx*ny*nz * %d steps / kernel time)\n",
+           nrun > 0 ? evo_ms / nrun : 0.0, mlups, nrun);
 
     FILE *fp = fopen((dir + "/runlog.out").c_str(), "w");
     if (fp) {
         fprintf(fp, "***\n*** Multiphase Zheng-Shu-Chew LBM 3D Simulation\n");
-        fprintf(fp, "*** MI355X HIP implementation (libzsc)\n***\n\n");
+        fprintf(fp, "*** MI355X HIP implementation (libzsc)\n***\n");
+        if (have_prop) {
+            const double gb = 1073741824.0, mb = 1048576.0, dm = (double)prop.totalGlobalMem;
+            fprintf(fp, "*** Program running on : %s\n", prop.name);
+            if (dm >= gb)
+                fprintf(fp, "*** Total GPU Memory   : %.4f GB\n", dm / gb);
+            else
+                fprintf(fp, "*** Total GPU Memory   : %.4f MB\n", dm / mb);
+            fprintf(fp, "***\n");
+        }
+        fprintf(fp, "*** Simulation started at %s\n***\n\n", started.c_str());
         fprintf(fp, "INPUT PARAMETERS\n");
         fprintf(fp, "Evolution Steps    = %d\n", in.max_step);
         fprintf(fp, "Relaxation Steps   = %d\n", in.relax_step);
@@ -278,15 +333,47 @@ int main(int argc, char **argv)
         fprintf(fp, "High Fluid Density = %6.3f\n", in.rho_h);
         fprintf(fp, "Tau (Density)      = %6.3f\n", in.tau_rho);
         fprintf(fp, "Tau (Phase)        = %6.3f\n\n", in.tau_phi);
-        if (resumed < 0) {
+        // launch configuration (the reference's CUDA CONFIGURATION section
+        // prints the properties.in block size its 2-D grid uses; here that
+        // value is read and printed, and the fused kernel's tile follows)
+        int ty = 0, zc = 0, fl = 0;
+        (void)zsc_get_fused_config(ctx, &ty, &zc, &fl);
+        const bool wide = (fl & 16) && prec != ZSC_F64 && in.xmax % 128 == 0 && (ty == 4 || ty == 8);
+        const int tile_x = wide ? 128 : 64;
+        fprintf(fp, "GPU CONFIGURATION\n");
+        fprintf(fp, "GPUs used    = %d\n", 1);
+        fprintf(fp, "BLOCK_SIZE_X = %d\n", in.block_x);
+        fprintf(fp, "BLOCK_SIZE_Y = %d\n", in.block_y);
+        fprintf(fp, "Fused tile   = %d x %d cells, %d-plane z chunks, flags %d\n", tile_x, ty,
+                zc > 0 ? zc : in.zmax, fl);
+        fprintf(fp, "Precision    = %s\n\n",
+                prec == ZSC_F64 ? "f64" : prec == ZSC_F32 ? "f32" : prec == ZSC_BF16 ? "bf16" : "fp16");
+        // device memory the context holds (two lattices, five moment arrays
+        // at the compute precision, an FP64 staging array) and the driver's
+        // host buffers (bubbles; VTK output streams through 2^22-cell slabs)
+        {
+            const double gb = 1073741824.0, mb = 1048576.0;
+            const double rsize = prec == ZSC_F64 ? 8.0 : 4.0;
+            const double dist = zsc_bytes_per_cell(prec) * cells;
+            const double aux = 5.0 * rsize * cells + 8.0 * cells;
+            const double cpu = (double)bub.size() * sizeof(double) +
+                               (vmode >= 0 ? 3.0 * (rsize + 36.0) * (double)(1 << 22) : 0.0);
+            const double gpu = dist + aux;
+            const double u = gpu < gb ? mb : gb;
+            fprintf(fp, "MEMORY USAGE (%s)\n", gpu < gb ? "Mb" : "Gb");
+            fprintf(fp, "Distributions         = %4.2f\n", dist / u);
+            fprintf(fp, "Auxiliary Arrays      = %4.2f\n", aux / u);
+            fprintf(fp, "Total CPU Memory Used = %4.2f\n", cpu / u);
+            fprintf(fp, "Total GPU Memory Used = %4.2f\n\n", gpu / u);
+        }
+        if (resumed >= 0) fprintf(fp, "RESTART\nResumed at step   = %lld\n\n", (long long)resumed);
+        if (resumed < 0 || n_user >= kUserCount) {
             fprintf(fp, "RELAXATION RESULTS\n");
             fprintf(fp, "Effective Radius   = %e\n", laplace[2]);
             fprintf(fp, "Phase Conservation = %e\n", relax_vef);
             fprintf(fp, "(Pin - Pout)       = %e\n", laplace[0]);
             fprintf(fp, "Laplace Error      = %e\n", laplace[1]);
             fprintf(fp, "Parasitic Velocity = %e\n\n", laplace[3]);
-        } else {
-            fprintf(fp, "RESTART\nResumed at step   = %lld\n\n", (long long)resumed);
         }
         const double vol = s.bubble_volume;
         fprintf(fp, "OUTPUT RESULTS\n");
@@ -299,8 +386,12 @@ int main(int argc, char **argv)
         fprintf(fp, "Setup Time      = %9.6f\n", t_setup * 1e-3);
         fprintf(fp, "Relaxation Time = %9.6f\n", t_relax * 1e-3);
         fprintf(fp, "Main Loop Time  = %9.6f\n", t_main * 1e-3);
+        fprintf(fp, "LBM step Time   = %9.6f\n", ref_step_ms * 1e-3);
+        fprintf(fp, "MLUPS           = %9.6f\n\n", ref_mlups);
+        fprintf(fp, "STEP KERNELS (nx*ny*nz cells, %d steps, kernel time only)\n", nrun);
         fprintf(fp, "LBM step Time   = %9.6f\n", nrun > 0 ? evo_ms * 1e-3 / nrun : 0.0);
-        fprintf(fp, "MLUPS           = %9.6f\n", mlups);
+        fprintf(fp, "MLUPS           = %9.6f\n\n", mlups);
+        fprintf(fp, "***\n*** Simulation completed at %s\n***\n", stamp().c_str());
         fclose(fp);
     }
     zsc_destroy(ctx);

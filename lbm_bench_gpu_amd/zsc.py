@@ -152,8 +152,21 @@ def _release_torch_cache() -> bool:
     return True
 
 
+def _alloc_call(fn, *args) -> int:
+    """Calls a libzsc entry point that may allocate device memory; on
+    ZSC_ERR_ALLOC, returns torch's cached blocks and tries once more.
+    libzsc allocates with hipMalloc, and blocks that torch has freed but
+    keeps cached are not available to it until torch returns them."""
+    rc = fn(*args)
+    if rc == _lib.ZSC_ERR_ALLOC and _release_torch_cache():
+        rc = fn(*args)
+    return check(rc)
+
+
 class Context:
-    """One periodic box on the current HIP device (zsc_create)."""
+    """One periodic box on the current HIP device (zsc_create, which makes no
+    device allocation; the lattices are allocated by the first call that
+    needs them)."""
 
     def __init__(self, nx: int, ny: int, nz: int, precision: str = "f64", stream=None):
         self.dims = (nx, ny, nz)
@@ -161,13 +174,7 @@ class Context:
         self.precision = precision
         self._h = ctypes.c_void_p()
         s = None if stream is None else ctypes.c_void_p(int(stream))
-        args = (nx, ny, nz, _lib.PRECISIONS[precision], s, ctypes.byref(self._h))
-        rc = lib().zsc_create(*args)
-        if rc == _lib.ZSC_ERR_ALLOC and _release_torch_cache():
-            # libzsc allocates with hipMalloc: blocks that torch has freed but
-            # keeps cached are not available to it until torch returns them
-            rc = lib().zsc_create(*args)
-        check(rc)
+        check(lib().zsc_create(nx, ny, nz, _lib.PRECISIONS[precision], s, ctypes.byref(self._h)))
 
     def close(self):
         if self._h:
@@ -195,6 +202,12 @@ class Context:
 
     def set_fused_config(self, ty: int, zchunk: int, flags: int = 0):
         check(lib().zsc_set_fused_config(self._h, ty, zchunk, flags))
+
+    def fused_config(self) -> tuple:
+        """(ty, zchunk, flags) of the fused kernel's launch shape in use."""
+        v = [ctypes.c_int(0) for _ in range(3)]
+        check(lib().zsc_get_fused_config(self._h, *map(ctypes.byref, v)))
+        return tuple(x.value for x in v)
 
     def sync(self):
         check(lib().zsc_sync(self._h))
@@ -245,11 +258,11 @@ class Context:
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         assert phi.size == self.n
         rho = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
-        check(lib().zsc_state_init(self._h, _dptr(phi), _dptr(rho)))
+        _alloc_call(lib().zsc_state_init, self._h, _dptr(phi), _dptr(rho))
 
     def state_init_bubbles(self, bubbles: np.ndarray):
         b = np.ascontiguousarray(bubbles, dtype=np.float64).reshape(-1, 4)
-        check(lib().zsc_state_init_bubbles(self._h, _dptr(b), b.shape[0]))
+        _alloc_call(lib().zsc_state_init_bubbles, self._h, _dptr(b), b.shape[0])
 
     def step(self, nsteps: int):
         check(lib().zsc_step(self._h, int(nsteps)))
@@ -263,26 +276,34 @@ class Context:
         """zsc_place_lattices: keeps the fastest of up to `trials` lattice
         allocations; returns each trial's ms per step (0: not run)."""
         ms = np.zeros(int(trials))
-        check(lib().zsc_place_lattices(self._h, int(trials), _dptr(ms)))
+        _alloc_call(lib().zsc_place_lattices, self._h, int(trials), _dptr(ms))
         return [float(x) for x in ms]
 
     def state_get(self):
         nx, ny, nz = self.dims
         f = np.empty((7, nz, ny, nx))
         g = np.empty((19, nz, ny, nx))
-        check(lib().zsc_state_get(self._h, _dptr(f), _dptr(g)))
+        _alloc_call(lib().zsc_state_get, self._h, _dptr(f), _dptr(g))
         return f, g
 
     def state_set(self, f: np.ndarray, g: np.ndarray):
         f = np.ascontiguousarray(f, dtype=np.float64)
         g = np.ascontiguousarray(g, dtype=np.float64)
-        check(lib().zsc_state_set(self._h, _dptr(f), _dptr(g)))
+        _alloc_call(lib().zsc_state_set, self._h, _dptr(f), _dptr(g))
 
     def moments(self):
         nx, ny, nz = self.dims
         out = [np.empty((nz, ny, nx)) for _ in range(5)]
-        check(lib().zsc_state_moments(self._h, *map(_dptr, out)))
+        _alloc_call(lib().zsc_state_moments, self._h, *map(_dptr, out))
         return tuple(out)
+
+    def moment_stats(self, phi, rho, ux, uy, uz) -> dict:
+        """zsc_moment_stats: the diagnostics of src/stats.c / src/logsave.c
+        over drop-in moment arrays (device tensors of N elements)."""
+        s = ZscStats()
+        _alloc_call(lib().zsc_moment_stats, self._h, *(self._ptr(t) for t in (phi, rho, ux, uy, uz)),
+                    ctypes.byref(s))
+        return {k: getattr(s, k) for k in STATS_FIELDS}
 
     def stats(self) -> dict:
         s = ZscStats()
@@ -293,25 +314,32 @@ class Context:
         """Pressure of the A state (src/vtksave.c:75-119), FP64 (nz, ny, nx)."""
         nx, ny, nz = self.dims
         out = np.empty((nz, ny, nx))
-        check(lib().zsc_state_pressure(self._h, _dptr(out)))
+        _alloc_call(lib().zsc_state_pressure, self._h, _dptr(out))
         return out
 
     def vtk_save(self, path: str, binary: bool = False):
         """Legacy VTK file of phi, pressure and velocity (src/vtksave.c)."""
-        check(lib().zsc_vtk_save(self._h, os.fsencode(path), 1 if binary else 0))
+        _alloc_call(lib().zsc_vtk_save, self._h, os.fsencode(path), 1 if binary else 0)
 
-    def save(self, path: str, step: int = 0):
-        """Restart file of the state as held; load() continues it bit for bit."""
-        check(lib().zsc_state_save(self._h, os.fsencode(path), int(step)))
+    def save(self, path: str, step: int = 0, values=()):
+        """Restart file of the state as held, with up to eight caller values
+        (zsc_state_save_ex); load() continues it bit for bit."""
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        check(lib().zsc_state_save_ex(self._h, os.fsencode(path), int(step),
+                                      _dptr(v) if v.size else None, int(v.size)))
 
-    def load(self, path: str) -> int:
-        """Restores state, parameters and gravity switch; returns the step."""
+    def load(self, path: str, with_values: bool = False):
+        """Restores state, parameters and gravity switch; returns the step,
+        or (step, caller values) with `with_values`."""
         st = ctypes.c_int64(0)
-        check(lib().zsc_state_load(self._h, os.fsencode(path), ctypes.byref(st)))
-        return st.value
+        buf = np.zeros(_lib.RESTART_USER_MAX)
+        nv = ctypes.c_int(0)
+        _alloc_call(lib().zsc_state_load_ex, self._h, os.fsencode(path), ctypes.byref(st),
+                    _dptr(buf), buf.size, ctypes.byref(nv))
+        return (st.value, buf[: nv.value].copy()) if with_values else st.value
 
     def state_export(self, f, g):
         check(lib().zsc_state_export(self._h, self._pops(f, 7), self._pops(g, 19)))
 
     def state_import(self, f, g):
-        check(lib().zsc_state_import(self._h, self._pops(f, 7), self._pops(g, 19)))
+        _alloc_call(lib().zsc_state_import, self._h, self._pops(f, 7), self._pops(g, 19))

@@ -93,3 +93,48 @@ def test_place_lattices_preconditions(need_gpu):
         ctx.set_params(p)
         ctx.state_init(phi0)
         assert _code(ctx.place_lattices, 2) == ZSC_ERR_STATE          # state already set
+
+
+def test_restart_file_validation_and_caller_values(need_gpu, tmp_path):
+    """zsc_state_save_ex / zsc_state_load_ex: caller values survive the round
+    trip; a file whose parameters were edited to eta = 1.5 or a non-finite
+    invEta2 is refused with ZSC_ERR_IO before the context's state is touched;
+    a version-1 file (no caller values) still loads to the same state."""
+    import struct
+    from lbm_bench_gpu_amd._lib import ZSC_ERR_IO, ZscParams
+    dims = (16, 8, 6)
+    p, phi0 = small_case(dims, seed=35)
+    path = str(tmp_path / "s.zsc")
+    with zsc.Context(*dims, precision="f64") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(3)
+        ctx.save(path, step=3, values=[1.5, -2.25])
+        f_ref, g_ref = ctx.state_get()
+    raw = bytearray(open(path, "rb").read())
+    assert struct.unpack_from("<i", raw, 8)[0] == 2                 # header version
+    hdr_v1 = 56 + ctypes.sizeof(ZscParams)                            # fixed part
+    hdr_v2 = hdr_v1 + 8 + 8 * 8                                       # + count, pad, 8 values
+    with zsc.Context(*dims, precision="f64") as ctx:
+        step, vals = ctx.load(path, with_values=True)
+        assert step == 3 and vals.tolist() == [1.5, -2.25]
+        f, g = ctx.state_get()
+        assert np.array_equal(f, f_ref) and np.array_equal(g, g_ref)
+        for field, value in (("eta", 1.5), ("inv_eta2", float("inf"))):
+            bad = bytearray(raw)
+            struct.pack_into("<d", bad, 56 + getattr(ZscParams, field).offset, value)
+            bpath = str(tmp_path / f"bad_{field}.zsc")
+            open(bpath, "wb").write(bad)
+            assert _code(ctx.load, bpath) == ZSC_ERR_IO, field
+            f, g = ctx.state_get()                                    # state untouched
+            assert np.array_equal(f, f_ref) and np.array_equal(g, g_ref)
+        v1 = bytearray(raw[:hdr_v1]) + raw[hdr_v2:]
+        struct.pack_into("<i", v1, 8, 1)
+        open(str(tmp_path / "v1.zsc"), "wb").write(v1)
+    with zsc.Context(*dims, precision="f64") as ctx:
+        step, vals = ctx.load(str(tmp_path / "v1.zsc"), with_values=True)
+        assert step == 3 and vals.size == 0
+        f, g = ctx.state_get()
+        assert np.array_equal(f, f_ref) and np.array_equal(g, g_ref)
+        ctx.step(2)
+        assert np.isfinite(ctx.stats()["sum_phi"])

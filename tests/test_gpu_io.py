@@ -225,6 +225,14 @@ def _driver(args, cwd):
     return r
 
 
+def _log_value(text, key):
+    """The number after `key ... =` on its line of a runlog.out."""
+    for line in text.splitlines():
+        if line.startswith(key):
+            return float(line.split("=")[1])
+    raise KeyError(key)
+
+
 def _vtk_phi(path, n):
     lines = open(path).read().split("\n")
     i = lines.index("SCALARS Phi float") + 2
@@ -252,7 +260,10 @@ def test_driver_run_outputs_and_restart(need_gpu, tmp_path):
     steps = [int(l.split()[0]) for l in open(d1 / "stats.out") if not l.startswith("#")]
     assert steps == [0, 5, 10, 15, 20]
     log = open(d1 / "runlog.out").read()
-    assert "RELAXATION RESULTS" in log and "Laplace Error" in log
+    for sec in ("*** Program running on", "*** Simulation started at", "INPUT PARAMETERS",
+                "GPU CONFIGURATION", "MEMORY USAGE", "RELAXATION RESULTS", "OUTPUT RESULTS",
+                "TIMING INFORMATION", "*** Simulation completed at"):
+        assert sec in log, sec
 
     inp = zsc.read_inputs(str(d1 / "properties.in"))
     b = zsc.read_bubbles(str(d1 / "discrete.in"))
@@ -260,6 +271,24 @@ def test_driver_run_outputs_and_restart(need_gpu, tmp_path):
     phi0 = O.init_phi((n, n, n), b, p.phi_star, p.width)
     st = O.State(phi0.shape, oracle_coef(p)).init(phi0)
     st.run(10, grav_on=False)
+    # the Laplace-law figures of src/logsave.c:155-167 from the oracle's
+    # reductions of the relaxed state (single static bubble of radius Ro)
+    s = O.stats(oracle_coef(p), *st.moments())
+    vin, vout = s[2], n ** 3 - s[2]
+    pdif = s[7] / vin - s[8] / vout
+    ro, sigma = float(b[0, 3]), p.sigma
+    want = {"(Pin - Pout)": pdif, "Laplace Error": 0.5 * (2 * sigma / ro - pdif) * ro / sigma,
+            "Effective Radius": (vin * 0.75 / np.pi) ** (1 / 3), "Parasitic Velocity": np.sqrt(s[6])}
+    relax = log.split("RELAXATION RESULTS")[1].split("\n\n")[0]
+    for key, v in want.items():
+        got = _log_value(relax, key)
+        print("runlog", key, got, v, flush=True)
+        assert got == pytest.approx(v, rel=2e-6, abs=1e-12), key
+    # the reference's MLUPS formula (src/logsave.c:262-263): maxStep steps of
+    # nx*ny*(nz+2) cells over the main-loop time
+    t_main = _log_value(log, "Main Loop Time")
+    mlups = _log_value(log.split("TIMING INFORMATION")[1], "MLUPS")
+    assert mlups == pytest.approx(21 * n * n * (n + 2) * 1e-6 / t_main, rel=1e-3)
     st.run(20, grav_on=True)
     got = _vtk_phi(d1 / "data_00_0000021.vtk", n ** 3)
     assert np.abs(got - st.moments()[0].ravel()).max() <= 1e-6  # "%f": 6 decimals
@@ -277,6 +306,22 @@ def test_driver_run_outputs_and_restart(need_gpu, tmp_path):
     key = b"SCALARS Phi float\nLOOKUP_TABLE default\n"
     got = np.frombuffer(raw, dtype=">f4", count=n ** 3, offset=raw.index(key) + len(key))
     assert np.array_equal(got, st.moments()[0].ravel().astype(np.float32))
+
+    # one uninterrupted 31-step run: the resumed run's stats lines (Vef is
+    # relative to the volume at the start of evolution, carried by the
+    # restart file) and its output and relaxation results are the same text
+    d3 = tmp_path / "run3"
+    d3.mkdir()
+    (d3 / "properties.in").write_text((d2 / "properties.in").read_text())
+    (d3 / "discrete.in").write_text((d1 / "discrete.in").read_text())
+    _driver(["-d", str(d3), "-p", "f64", "-v", "none"], str(d3))
+    full = {l.split()[0]: l for l in open(d3 / "stats.out") if not l.startswith("#")}
+    for l in open(d2 / "stats.out"):
+        if not l.startswith("#"):
+            assert l == full[l.split()[0]], (l, full[l.split()[0]])
+    log2, log3 = open(d2 / "runlog.out").read(), open(d3 / "runlog.out").read()
+    for sec in ("RELAXATION RESULTS", "OUTPUT RESULTS"):
+        assert log2.split(sec)[1].split("\n\n")[0] == log3.split(sec)[1].split("\n\n")[0], sec
 
 
 @pytest.mark.parametrize("prec,tol", [("f32", 1e-4), ("bf16", 2e-3), ("fp16", 2e-4)])

@@ -222,6 +222,45 @@ def test_stats_match_oracle(need_gpu):
         assert abs(s[k] - r) <= 1e-10 * max(1.0, abs(r)), (k, s[k], r)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_moment_stats_on_dropin_arrays(need_gpu, prec):
+    """zsc_moment_stats over the moment arrays the drop-in update kernels
+    leave (src/main.cu:498-503 then src/stats.c): equal to the oracle's
+    reductions of the same state, and for FP64 bit for bit equal to
+    zsc_state_stats of that state imported into the fused path."""
+    import torch
+    dims = (24, 20, 16)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, seed=2)
+    st = run_oracle(p, phi0, 10)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    f, g, mom = _dropin_tensors(n, dt)
+    with zsc.Context(*dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.step(10)
+        ctx.state_export(f, g)
+        phi, rho, ux, uy, uz = mom
+        ctx.update_phi(phi, f)
+        ctx.update_rho(rho, g)
+        ctx.update_velocity(rho, ux, uy, uz, g)
+        s = ctx.moment_stats(*mom)
+        s_state = ctx.stats()
+        ctx.sync()
+        host = [m.cpu().numpy().astype(np.float64).reshape(nz, ny, nx) for m in mom]
+    # the oracle's reductions of the very arrays reduced on the device
+    for k, r in zip(STATS_NAMES, O.stats(oracle_coef(p), *host)):
+        assert abs(s[k] - r) <= 1e-10 * max(1.0, abs(r)), (k, s[k], r)
+    if prec == "f64":
+        assert all(np.array_equal(a, b) for a, b in zip(host, st.moments()))
+        assert s == s_state
+
+
+STATS_NAMES = ["sum_phi", "sum_rho", "bubble_volume", "ux_sum", "uy_sum", "uz_sum", "umax_sq",
+               "p_in_sum", "p_out_sum", "bubble_volume_full"]
+
+
 def test_bubble_init_on_device_matches_host(need_gpu):
     dims = (48, 40, 36)
     nx, ny, nz = dims

@@ -63,6 +63,11 @@ def test_overrides_and_errors(tmp_path):
         "no_value": (src + "\nphiStar\n", _lib.ZSC_ERR_IO),
         "tau_half": (src.replace("1.00\t0.7", "0.50\t0.7"), _lib.ZSC_ERR_ARG),
         "zero_size": (src.replace("128\t128\t128", "0\t128\t128"), _lib.ZSC_ERR_ARG),
+        # integers outside the range of int (2^32 + 24 would wrap to 24)
+        "size_wraps": (src.replace("128\t128\t128", "4294967320\t128\t128"), _lib.ZSC_ERR_IO),
+        "size_erange": (src.replace("128\t128\t128", "99999999999999999999999\t128\t128"),
+                        _lib.ZSC_ERR_IO),
+        "steps_negative_wrap": (src.replace("1000", "-4294967295", 1), _lib.ZSC_ERR_IO),
     }
     for name, (text, code) in cases.items():
         path = tmp_path / f"{name}.in"
@@ -74,6 +79,43 @@ def test_overrides_and_errors(tmp_path):
     with pytest.raises(_lib.ZscError) as e:
         zsc.read_bubbles(str(tmp_path / "nope.in"))
     assert e.value.code == _lib.ZSC_ERR_IO
+    # a bubble count beyond int is rejected before anything is sized from it
+    big = tmp_path / "many.in"
+    big.write_text("nBubbles\n4294967297\n\nxc\tyc\tzc\tRc\n1\t1\t1\t1\n")
+    with pytest.raises(_lib.ZscError) as e:
+        zsc.read_bubbles(str(big))
+    assert e.value.code == _lib.ZSC_ERR_IO
+
+
+def test_allocating_calls_retry_once_after_releasing_torch_cache(monkeypatch):
+    """An entry point that fails with ZSC_ERR_ALLOC is called again once after
+    torch's cached blocks are returned, and the error stands when there was
+    nothing to return or the second call fails too."""
+    calls = []
+
+    def fake(rcs):
+        def fn(*args):
+            calls.append(args)
+            return rcs[len(calls) - 1]
+        return fn
+
+    monkeypatch.setattr(zsc, "_release_torch_cache", lambda: True)
+    assert zsc._alloc_call(fake([_lib.ZSC_ERR_ALLOC, _lib.ZSC_OK]), 1, 2) == _lib.ZSC_OK
+    assert calls == [(1, 2), (1, 2)]
+    calls.clear()
+    with pytest.raises(_lib.ZscError) as e:
+        zsc._alloc_call(fake([_lib.ZSC_ERR_ALLOC, _lib.ZSC_ERR_ALLOC]))
+    assert e.value.code == _lib.ZSC_ERR_ALLOC and len(calls) == 2
+    calls.clear()
+    monkeypatch.setattr(zsc, "_release_torch_cache", lambda: False)
+    with pytest.raises(_lib.ZscError):
+        zsc._alloc_call(fake([_lib.ZSC_ERR_ALLOC, _lib.ZSC_OK]))
+    assert len(calls) == 1
+    calls.clear()
+    monkeypatch.setattr(zsc, "_release_torch_cache", lambda: True)
+    with pytest.raises(_lib.ZscError) as e:   # other codes are not retried
+        zsc._alloc_call(fake([_lib.ZSC_ERR_ARG, _lib.ZSC_OK]))
+    assert e.value.code == _lib.ZSC_ERR_ARG and len(calls) == 1
 
 
 def test_bubble_file_round_trip(tmp_path):
