@@ -241,6 +241,19 @@ int zsc_state_stats(zsc_ctx *ctx, zsc_stats *out);
  * partial sums on first use. */
 int zsc_moment_stats(zsc_ctx *ctx, const void *phi, const void *rho, const void *ux,
                      const void *uy, const void *uz, zsc_stats *out);
+/* zsc_moment_stats without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a library-owned device array of 82 KB per GPU
+ * ((1024 + 1) * 10 doubles, held until the process ends, its address looked
+ * up once per device by zsc_create).  The reduction's grid is fixed, so
+ * every box a context accepts fits; each call overwrites all of the array
+ * it reads, so nothing needs clearing between calls.  Calls that overlap on
+ * one GPU (several host threads, or contexts on several streams) share that
+ * array and race: serialise them.  Results equal zsc_moment_stats' bit for
+ * bit (same kernels, same order). */
+int zsc_moment_stats_noalloc(zsc_ctx *ctx, const void *phi, const void *rho, const void *ux,
+                             const void *uy, const void *uz, zsc_stats *out);
 
 /* ---- output and restart ------------------------------------------------ */
 /* Pressure of src/vtksave.c:75-119 (periodic 19-point stencil of phi, at the

@@ -90,6 +90,7 @@ SIGNATURES = {
     "zsc_state_import": (_I, [_VP, _PP7, _PP19]),
     "zsc_state_stats": (_I, [_VP, ctypes.POINTER(ZscStats)]),
     "zsc_moment_stats": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(ZscStats)]),
+    "zsc_moment_stats_noalloc": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(ZscStats)]),
     "zsc_state_pressure": (_I, [_VP, _DP]),
     "zsc_vtk_save": (_I, [_VP, ctypes.c_char_p, _I]),
     "zsc_state_save": (_I, [_VP, ctypes.c_char_p, ctypes.c_int64]),

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -86,6 +87,7 @@ struct zsc_ctx {
     size_t rsize = 8;
     double *dbuf = nullptr;  // n doubles of staging for host transfers
     double *stats_buf = nullptr;
+    double *scratch = nullptr;  // this device's copy of the no-allocation scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     FusedCfg cfg{4, 16, 0};
     // deferred coefficient sums (backward calls with grads == NULL) exist
@@ -107,6 +109,36 @@ static int set_device(zsc_ctx *c)
 }
 
 extern "C" const char *zsc_last_error(void) { return g_last_error.c_str(); }
+
+// Results (10 doubles) and per-block partial sums of the statistics
+// reduction: a fixed grid of kStatsBlocks blocks strides over any box.
+static constexpr size_t kStatsScratch = (size_t)(kStatsBlocks + 1) * 10;
+
+// ---- the same reduction with its scratch in a file-scope device array ----
+// For callers that cannot allocate device memory (code under graph capture,
+// serving loops, harnesses that forbid hipMalloc).  Each GPU the library runs
+// on holds one copy of `scratch` (kStatsScratch doubles, 82 KB) until the
+// process ends.  In host code the array's name is not a device address: each
+// device's copy is looked up once, with that device current, when the first
+// context on it is created (zsc_create), and only that pointer is used.
+static __device__ __align__(256) double scratch[kStatsScratch];
+static std::mutex g_scratch_mu;
+static std::vector<double *> g_scratch_ptr;  // by device ordinal
+
+static int scratch_for_device(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return fail(ZSC_ERR_ARG, "bad device %d", device);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return ZSC_OK;
+}
 
 extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, zsc_ctx **out)
 {
@@ -153,6 +185,12 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     if (e != hipSuccess) {
         zsc_destroy(c);
         return hip_fail(e, "zsc_create");
+    }
+    // this device's copy of the no-allocation scratch (looked up once per
+    // device, with the device current)
+    if (int r = scratch_for_device(c->device, &c->scratch)) {
+        zsc_destroy(c);
+        return r;
     }
     *out = c;
     return ZSC_OK;
@@ -383,10 +421,6 @@ static int ensure_alloc(zsc_ctx *c)
     c->work = w;
     return ZSC_OK;
 }
-
-// Results (10 doubles) and per-block partial sums of the statistics
-// reduction: a fixed grid of kStatsBlocks blocks strides over any box.
-static constexpr size_t kStatsScratch = (size_t)(kStatsBlocks + 1) * 10;
 
 static int ensure_stats_buf(zsc_ctx *c)
 {
@@ -827,6 +861,23 @@ extern "C" int zsc_state_export(zsc_ctx *c, void *const f[7], void *const g[19])
 extern "C" int zsc_state_import(zsc_ctx *c, void *const f[7], void *const g[19])
 {
     return xport<false>(c, f, g, "zsc_state_import");
+}
+
+extern "C" int zsc_moment_stats_noalloc(zsc_ctx *c, const void *phi, const void *rho,
+                                        const void *ux, const void *uy, const void *uz,
+                                        zsc_stats *out)
+{
+    if (int r = need_ctx(c)) return r;
+    if (!moments_set(phi, rho, ux, uy, uz, out))
+        return fail(ZSC_ERR_ARG, "zsc_moment_stats_noalloc: null pointer");
+    if (c->prec != ZSC_F64 && c->prec != ZSC_F32)
+        return fail(ZSC_ERR_PRECISION, "zsc_moment_stats_noalloc: drop-in arrays need an F64 or F32 context");
+    // the reduction's grid is fixed, so the scratch it needs does not grow
+    // with the box: every box a context accepts fits
+    static_assert(sizeof(scratch) / sizeof(double) >= kStatsScratch, "scratch too small");
+    if (!c->scratch)
+        return fail(ZSC_ERR_STATE, "zsc_moment_stats_noalloc: no scratch address for device %d", c->device);
+    return moment_stats(c, phi, rho, ux, uy, uz, out, c->scratch, "zsc_moment_stats_noalloc");
 }
 
 // ---------------------------------------------------------------------------

@@ -297,12 +297,16 @@ class Context:
         _alloc_call(lib().zsc_state_moments, self._h, *map(_dptr, out))
         return tuple(out)
 
-    def moment_stats(self, phi, rho, ux, uy, uz) -> dict:
+    def moment_stats(self, phi, rho, ux, uy, uz, noalloc: bool = False) -> dict:
         """zsc_moment_stats: the diagnostics of src/stats.c / src/logsave.c
-        over drop-in moment arrays (device tensors of N elements)."""
+        over drop-in moment arrays (device tensors of N elements); with
+        `noalloc`, zsc_moment_stats_noalloc, which allocates nothing."""
         s = ZscStats()
-        _alloc_call(lib().zsc_moment_stats, self._h, *(self._ptr(t) for t in (phi, rho, ux, uy, uz)),
-                    ctypes.byref(s))
+        args = (self._h, *(self._ptr(t) for t in (phi, rho, ux, uy, uz)), ctypes.byref(s))
+        if noalloc:
+            check(lib().zsc_moment_stats_noalloc(*args))
+        else:
+            _alloc_call(lib().zsc_moment_stats, *args)
         return {k: getattr(s, k) for k in STATS_FIELDS}
 
     def stats(self) -> dict:
