@@ -54,10 +54,15 @@ def parse():
                          "7-kernel step through the drop-in entry points at 256^3 FP32, "
                          "share = one GPU's z-slab of the 1024^3 box (1024x1024x128) at "
                          "FP32 and bf16 (empty disables)")
-    ap.add_argument("--copy-probe", type=int, default=4096, metavar="MIB",
-                    help="time a plain device copy of MIB MiB: the practical read+write "
-                         "ceiling reported beside the roofline (0 disables)")
     return ap.parse_args()
+
+
+# what each precision stores (compute is FP32 in all but f64)
+STORAGE = {
+    "f32": "f, g fp32 (208 B/cell)",
+    "bf16": "g_i - w_i rho bf16, f_1..6 fp16, f_0 fp32 (mixed 16-bit storage, 108 B/cell)",
+    "fp16": "g_i - w_i rho fp16, f_1..6 fp16, f_0 fp32 (108 B/cell)",
+}
 
 
 def configure(ctx, ty, zchunk, flags):
@@ -81,6 +86,8 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags, trials=0):
             tp = time.perf_counter()
             placement = [round(m, 4) for m in ctx.place_lattices(trials)]
             placement_s = time.perf_counter() - tp
+        # read + write ceiling of the pages the timed lattices sit on
+        ceiling = ctx.copy_ceiling()
         ctx.state_init_bubbles(w.bubbles)
         ctx.step(warmup + 1)  # the first step also runs the A-lattice prologue
         ctx.sync()
@@ -96,13 +103,25 @@ def time_precision(prec, w, params, steps, warmup, ty, zchunk, flags, trials=0):
     bpc = zsc.bytes_per_cell(prec)
     per_launch = ev_ms / 1e3 / steps
     achieved = bpc * cells / per_launch / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_cell": bpc}
+    if placement:
+        # the step's rate at each tried allocation (4 steps each); the timed
+        # run uses the fastest, and the median says what one allocation gives
+        fr = sorted(bpc * cells / (m * 1e-3) / 1e9 / HBM_PEAK_GBS for m in placement if m > 0)
+        roof["frac_placement_median"] = round(float(np.median(fr)), 4)
+        roof["frac_placement_range"] = [round(fr[0], 4), round(fr[-1], 4)]
+    for k in ("linear", "march"):
+        gbs = ceiling[k]["GB/s"]
+        roof[f"copy_{k}_GBs"] = gbs
+        if gbs:
+            roof[f"frac_of_copy_{k}"] = round(achieved / gbs, 4)
     return {
         "mlups": cells * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms": per_launch * 1e3,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_cell": bpc},
+        "roofline": roof,
         "sum_phi": st["sum_phi"],
         "finite": bool(np.isfinite(st["sum_phi"])),
         "placement_ms": placement,
@@ -219,11 +238,15 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     return out
 
 
+TRAFFIC_FILE = "profiles/k_fused_{}x{}x{}_traffic.txt"
+
+
 def traffic_from_profile(dims):
-    """HBM bytes per fused launch per precision, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this box (empty when absent)."""
-    nx, ny, nz = dims
-    path = os.path.join(ROOT, "profiles", f"k_fused_{nx}x{ny}x{nz}_traffic.txt")
+    """HBM bytes per fused launch per precision as counted by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE of this box, scaled by a copy calibration), read
+    from the committed summary: a profiler cannot run inside this process,
+    so the figure is not measured in this run (empty when absent)."""
+    path = os.path.join(ROOT, TRAFFIC_FILE.format(*dims))
     if not os.path.exists(path):
         return {}
     out = {}
@@ -234,22 +257,15 @@ def traffic_from_profile(dims):
     return out
 
 
-def copy_probe(mib: int):
-    """A plain device copy (torch copy_ kernel) of a known size: the
-    reference point for the FETCH_SIZE / WRITE_SIZE counter scale."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / 5
-    return {"bytes_read": 4 * n, "bytes_written": 4 * n, "ms": round(dt * 1e3, 4),
-            "GB/s": round(8 * n / dt / 1e9, 1)}
+def cpu_model() -> str:
+    """The host CPU's model name (what lscpu prints), from /proc/cpuinfo."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def workload_1(steps: int):
@@ -401,7 +417,7 @@ def cpu_baseline(seconds: float, w):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     del st
     return {"value": round(cells * k / dt / 1e6, 3), "unit": "MLUPS", "cores": cores,
-            "kind": "port",
+            "cpu_model": cpu_model(), "kind": "port",
             "sample": f"FP64 oracle (C, OpenMP, {cores} threads) on a {nx}x{ny}x{nzs} z-slab of "
                       f"the {w.name} initial state, {k} steps in {dt:.1f} s",
             "workloads": cpu_side_by_side()}
@@ -507,16 +523,13 @@ def main():
             extra_w[wid] = workload_share(200, a.ty, a.zchunk, a.flags, a.placement_trials)
         else:
             raise SystemExit(f"no extra workload {wid}")
-    probe = copy_probe(a.copy_probe) if a.copy_probe > 0 else None
     cpu = cpu_baseline(a.cpu_seconds, w) if a.cpu_seconds > 0 else None
     traffic = traffic_from_profile(w.dims)
-    main_r["roofline"]["traffic"] = traffic.get("f32")
-    for k, v in extra.items():
-        v["roofline"]["traffic"] = traffic.get(k)
-    if probe:
-        for r in [main_r] + list(extra.values()):
-            r["roofline"]["copy_GBs"] = probe["GB/s"]
-            r["roofline"]["frac_of_copy"] = round(r["roofline"]["achieved"] / probe["GB/s"], 4)
+    for k, r in [("f32", main_r)] + list(extra.items()):
+        r["roofline"]["traffic"] = traffic.get(k)
+        if traffic.get(k) is not None:
+            r["roofline"]["traffic_source"] = (TRAFFIC_FILE.format(*w.dims) + ": rocprofv3 --pmc "
+                                               "FETCH_SIZE, WRITE_SIZE of this command, not this run")
     out = {
         "metric": "MLUPS + %HBM roofline, FP32 vs bf16-store (err vs FP64); adjoint MLUPS",
         "value": round(main_r["mlups"], 1),
@@ -531,19 +544,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "cells": w.cells, "bubbles": int(len(w.bubbles)),
-                   "precision": "fp32 storage, fp32 compute", "gravity": "on"},
+                   "precision": "fp32 compute; " + STORAGE["f32"], "gravity": "on"},
         "roofline": main_r["roofline"],
         "kernel_ms": round(main_r["kernel_ms"], 4),
         "placement_ms": main_r["placement_ms"],
         "placement_s": main_r["placement_s"],
-        "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
+        "variants": {k: {"storage": STORAGE.get(k, k),
+                         "mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                          "speedup_vs_fp32": round(v["mlups"] / main_r["mlups"], 3),
                          "roofline": v["roofline"], "placement_ms": v["placement_ms"]}
                      for k, v in extra.items()},
         "accuracy_vs_fp64": acc,
         "adjoint": adj,
         "workloads": extra_w,
-        "copy_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

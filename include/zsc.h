@@ -349,6 +349,19 @@ int zsc_adj_flush(zsc_ctx *ctx, void *scratch, double *grads);
 int zsc_adj_init_backward(zsc_ctx *ctx, const void *phi0, const void *lam0, void *scratch,
                           double *grads);
 
+/* The practical read + write ceiling of the memory behind the context's two
+ * lattices: device time (ms per copy, mean of five) of plain copies from one
+ * lattice to the other, each moving 2 x the lattice bytes.  ms[0]: a linear
+ * copy (each lane moves 64 bytes, loads before stores); ms[1]: the fused
+ * step's access pattern without its neighbour pulls or arithmetic (tiles
+ * marching z, the 26 populations of each cell; 0 when the x extent does not
+ * suit the 16-bit two-cell pattern).  The step's time depends on the
+ * physical pages behind the lattices, so this is measured on the same ones:
+ * call it after zsc_set_params / zsc_place_lattices and before the state is
+ * set (it overwrites the lattices; ZSC_ERR_STATE otherwise).  No reference
+ * counterpart: roofline accounting. */
+int zsc_copy_ceiling(zsc_ctx *ctx, double ms[2]);
+
 /* Bytes per lattice update the fused step must move (26 populations read
  * and written once at the storage precision).  No reference counterpart:
  * roofline accounting. */

@@ -99,6 +99,7 @@ SIGNATURES = {
     "zsc_state_load_ex": (_I, [_VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), _DP, _I,
                                ctypes.POINTER(_I)]),
     "zsc_bytes_per_cell": (_D, [_I]),
+    "zsc_copy_ceiling": (_I, [_VP, _DP]),
     "zsc_adj_scratch_bytes": (ctypes.c_size_t, [_VP]),
     "zsc_adj_init": (_I, [_VP, _VP, _VP]),
     "zsc_adj_forward": (_I, [_VP, _VP, _VP, _VP]),

@@ -881,6 +881,117 @@ extern "C" int zsc_moment_stats_noalloc(zsc_ctx *c, const void *phi, const void 
 }
 
 // ---------------------------------------------------------------------------
+// copy ceiling: plain copies between the context's two lattices
+// ---------------------------------------------------------------------------
+typedef float zsc_v4f __attribute__((ext_vector_type(4)));
+
+// Linear copy: each lane loads four 16-byte vectors of its block's 16 KiB,
+// then stores them.
+__global__ void __launch_bounds__(256) k_copy_linear(const zsc_v4f *__restrict__ in,
+                                                     zsc_v4f *__restrict__ out)
+{
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    zsc_v4f v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = __builtin_nontemporal_load(in + base + k * 256);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], out + base + k * 256);
+}
+
+__global__ void __launch_bounds__(256) k_copy_tail(const unsigned char *in, unsigned char *out,
+                                                   size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        out[i] = in[i];
+}
+
+// The fused step's memory pattern without its pulls or arithmetic: tiles of
+// 64 lane-words x TY rows march a z chunk and move the 26 populations of
+// each cell (a lane-word holds one cell, or two for 16-bit storage: W0 is
+// the f_0 word, W1 the word of the other 25 rows).  XCD-contiguous tile
+// order, non-temporal stores, as in the fused kernel.
+template <class W0, class W1, int TY>
+__global__ void __launch_bounds__(64 * TY)
+    k_copy_march(const W0 *__restrict__ in0, W0 *__restrict__ out0, const W1 *__restrict__ in1,
+                 W1 *__restrict__ out1, int nxw, int ny, int nz, int zchunk, int nbx, int nby,
+                 int nblk, int64_t row1)
+{
+    int l = blockIdx.x;
+    if ((nblk & 7) == 0) l = (l & 7) * (nblk >> 3) + (l >> 3);
+    const int bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
+    const int x = bx * 64 + (int)threadIdx.x, y = by * TY + (int)threadIdx.y;
+    if (x >= nxw || y >= ny) return;
+    const int z1 = min(nz, (bz + 1) * zchunk);
+    for (int z = bz * zchunk; z < z1; ++z) {
+        const int64_t o = x + (int64_t)nxw * (y + (int64_t)ny * z);
+        const W0 a = in0[o];
+        W1 v[25];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) v[q] = in1[o + q * row1];
+        __builtin_nontemporal_store(a, out0 + o);
+#pragma unroll
+        for (int q = 0; q < 25; ++q) __builtin_nontemporal_store(v[q], out1 + o + q * row1);
+    }
+}
+
+template <class W0, class W1, int CPL>
+static hipError_t launch_copy_march(const Geo &g, hipStream_t s, size_t f_off, const void *a, void *b)
+{
+    constexpr int TY = CPL == 1 ? 8 : 4;
+    const int nxw = g.nx / CPL, zchunk = 16;
+    const int nbx = (nxw + 63) / 64, nby = (g.ny + TY - 1) / TY, nbz = (g.nz + zchunk - 1) / zchunk;
+    const int64_t nblk = (int64_t)nbx * nby * nbz;
+    if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_copy_march<W0, W1, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
+                       (const W0 *)a, (W0 *)b, (const W1 *)((const char *)a + f_off),
+                       (W1 *)((char *)b + f_off), nxw, g.ny, g.nz, zchunk, nbx, nby, (int)nblk,
+                       (int64_t)(g.n / CPL));
+    return hipGetLastError();
+}
+
+extern "C" int zsc_copy_ceiling(zsc_ctx *c, double ms[2])
+{
+    if (int r = need_ctx(c)) return r;
+    if (!ms) return fail(ZSC_ERR_ARG, "zsc_copy_ceiling: null output");
+    if (c->state != STATE_NONE)
+        return fail(ZSC_ERR_STATE, "zsc_copy_ceiling: overwrites the lattices; call it before the state is set");
+    if (int r = ensure_alloc(c)) return r;
+    const void *a = c->lat[0];
+    void *b = c->lat[1];
+    const size_t bytes = c->lat_bytes, chunks = bytes / 16384, done = chunks * 16384;
+    auto linear = [&]() {
+        if (chunks) hipLaunchKernelGGL(k_copy_linear, dim3((unsigned)chunks), dim3(256), 0, c->stream,
+                                       (const zsc_v4f *)a, (zsc_v4f *)b);
+        if (bytes > done)
+            hipLaunchKernelGGL(k_copy_tail, dim3(64), dim3(256), 0, c->stream,
+                               (const unsigned char *)a + done, (unsigned char *)b + done, bytes - done);
+        return hipGetLastError();
+    };
+    const bool wide = c->prec == ZSC_BF16 || c->prec == ZSC_FP16;
+    auto march = [&]() {
+        if (c->prec == ZSC_F64) return launch_copy_march<double, double, 1>(c->geo, c->stream, c->geo.n * 8, a, b);
+        if (c->prec == ZSC_F32) return launch_copy_march<float, float, 1>(c->geo, c->stream, c->geo.n * 4, a, b);
+        return launch_copy_march<uint64_t, uint32_t, 2>(c->geo, c->stream, c->geo.n * 4, a, b);
+    };
+    if (chunks > 0x7fffffff) return fail(ZSC_ERR_ARG, "zsc_copy_ceiling: lattice too large");
+    const bool can_march = !wide || c->geo.nx % 2 == 0;
+    for (int k = 0; k < 2; ++k) {
+        ms[k] = 0.0;
+        if (k == 1 && !can_march) continue;
+        auto run = [&]() { return k == 0 ? linear() : march(); };
+        ZSC_HIP(run(), "copy");  // warm-up
+        ZSC_HIP(hipEventRecord(c->ev0, c->stream), "event");
+        for (int r = 0; r < 5; ++r) ZSC_HIP(run(), "copy");
+        ZSC_HIP(hipEventRecord(c->ev1, c->stream), "event");
+        ZSC_HIP(hipEventSynchronize(c->ev1), "event sync");
+        float t = 0.f;
+        ZSC_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1), "elapsed");
+        ms[k] = t / 5.0;
+    }
+    return ZSC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // pressure, VTK output and restart files
 // ---------------------------------------------------------------------------
 // Moments of the A state with the pressure over the density: w(0) phi,

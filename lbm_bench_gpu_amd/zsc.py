@@ -279,6 +279,19 @@ class Context:
         _alloc_call(lib().zsc_place_lattices, self._h, int(trials), _dptr(ms))
         return [float(x) for x in ms]
 
+    def copy_ceiling(self) -> dict:
+        """zsc_copy_ceiling on the context's own lattice pair (before the
+        state is set): ms per copy and GB/s of read + write traffic for a
+        linear copy and for the fused step's access pattern."""
+        ms = np.zeros(2)
+        _alloc_call(lib().zsc_copy_ceiling, self._h, _dptr(ms))
+        lat = bytes_per_cell(self.precision) / 2 * self.n
+        out = {"bytes_per_copy": 2 * lat}
+        for name, t in zip(("linear", "march"), ms):
+            out[name] = {"ms": round(float(t), 4),
+                         "GB/s": round(2 * lat / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+        return out
+
     def state_get(self):
         nx, ny, nz = self.dims
         f = np.empty((7, nz, ny, nx))

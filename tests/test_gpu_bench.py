@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_bench_json_line(need_gpu):
     args = ["--steps", "3", "--warmup", "1", "--size", "128", "--variants", "bf16",
             "--accuracy-steps", "0", "--adjoint-steps", "0", "--cpu-seconds", "1",
-            "--workloads", "", "--copy-probe", "64", "--placement-trials", "2"]
+            "--workloads", "", "--placement-trials", "2"]
     print("bench.py", " ".join(args), flush=True)
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, cwd=ROOT,
                        capture_output=True, text=True, timeout=300)
@@ -34,5 +34,14 @@ def test_bench_json_line(need_gpu):
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("port", "reference")
     assert len(d["placement_ms"]) == 2 and min(d["placement_ms"]) > 0 and d["placement_s"] > 0
+    lo, hi = rf["frac_placement_range"]
+    assert lo <= rf["frac_placement_median"] <= hi
+    # copy ceilings on the timed lattice pair, and the step's share of them
+    for k in ("linear", "march"):
+        assert rf[f"copy_{k}_GBs"] > 0
+        assert abs(rf[f"frac_of_copy_{k}"] - rf["achieved"] / rf[f"copy_{k}_GBs"]) < 1e-3
+    assert cb["cpu_model"]
     v = d["variants"]["bf16"]
     assert v["mlups"] > 0 and 0 < v["roofline"]["frac"] < 1
+    assert "fp16" in v["storage"] and "bf16" in v["storage"]
+    assert v["roofline"]["copy_march_GBs"] > 0

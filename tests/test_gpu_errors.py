@@ -76,6 +76,24 @@ def test_diverged_state_is_reported(need_gpu, prec):
         assert np.isfinite(ctx.stats()["sum_rho"])
 
 
+@pytest.mark.parametrize("prec,dims", [("f32", (64, 16, 8)), ("bf16", (128, 8, 8)),
+                                       ("fp16", (63, 5, 3)), ("f64", (16, 8, 4))])
+def test_copy_ceiling(need_gpu, prec, dims):
+    """zsc_copy_ceiling times both copies on the context's lattices (the
+    march copy only where the x extent suits its lane words) and refuses to
+    run once a state is set, which it would overwrite."""
+    p, phi0 = small_case(dims, seed=36)
+    with zsc.Context(*dims, precision=prec) as ctx:
+        ctx.set_params(p)
+        c = ctx.copy_ceiling()
+        assert c["linear"]["ms"] > 0 and c["bytes_per_copy"] > 0
+        assert (c["march"]["ms"] > 0) == (prec in ("f32", "f64") or dims[0] % 2 == 0)
+        ctx.state_init(phi0)
+        assert _code(ctx.copy_ceiling) == ZSC_ERR_STATE
+        ctx.step(1)
+        assert np.isfinite(ctx.stats()["sum_phi"])
+
+
 def test_place_lattices_preconditions(need_gpu):
     dims = (16, 8, 4)
     p, phi0 = small_case(dims, seed=34)
