@@ -201,7 +201,11 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     out = {"workload": w.name, "steps": steps, "precision": "f32",
            "fused_forward_mlups": round(fused, 1)}
     grads = {}
-    for mode, kw in (("tape", {"tape": "auto"}), ("checkpoints", {"tape": False})):
+    modes = (("tape", {"tape": "auto"}), ("checkpoints", {"tape": False}),
+             # FP32 adjoint lattices over FP64 states: the FP64 forward's
+             # gradients (1e-7 of central differences) at FP32 adjoint storage
+             ("checkpoints_f32_over_f64_states", {"tape": False, "states": "f64"}))
+    for mode, kw in modes:
         run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20, timing=True,
                                     **kw)
         tgt = run.forward(tt, phi0, steps, keep=False)[:7].sum(0).clone()
@@ -216,7 +220,8 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
         adj = w.cells * steps / (t3 - t2) / 1e6
         tm = run.last_timing
         adj_steps = w.cells * steps / (tm["adjoint_steps_ms"] / 1e3) / 1e6
-        r = {"checkpoint_every": 1 if run.tape is not None else 20,
+        r = {"states": kw.get("states", "f32"), "adjoint_lattices": "f32",
+             "checkpoint_every": 1 if run.tape is not None else 20,
              "tape_GB": round(run.tape_bytes(steps) / 1e9, 1) if run.tape is not None else 0,
              "forward_mlups": round(w.cells * steps / (t1 - t0) / 1e6, 1),
              "adjoint_mlups": round(adj, 1), "adjoint_vs_forward": round(adj / fused, 3),
@@ -231,6 +236,8 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
                         "adjoint_step_vs_forward": r["adjoint_step_vs_forward"],
                         "adjoint_mlups": r["adjoint_mlups"],
                         "adjoint_vs_forward": r["adjoint_vs_forward"]})
+        if "states" in kw:
+            r["grad"] = dict(zip(adjoint.THETA, grads[mode]))
         out[mode] = r
         del run, a, lam
         torch.cuda.empty_cache()

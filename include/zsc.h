@@ -315,6 +315,16 @@ int zsc_get_fused_config(zsc_ctx *ctx, int *ty, int *zchunk, int *flags);
  * zsc_step.  A context's scratch buffer of zsc_adj_scratch_bytes() is passed
  * in by the caller, which owns all memory. */
 size_t zsc_adj_scratch_bytes(zsc_ctx *ctx);
+/* Storage of the adjoint lattices (lam arguments of zsc_adj_seed,
+ * zsc_adj_backward, zsc_adj_init_backward): the context's own precision
+ * (default), or ZSC_F32 over an F64 context -- FP32 adjoint lattices over
+ * FP64 states, with the per-cell reverse arithmetic and the coefficient
+ * sums in FP64.  That mode keeps the FP64 forward's accuracy where FP32
+ * states cannot: the tau_g derivative pairs sum_i lam_i g_i against
+ * equilibria built from rho = sum_i g_i, whose FP32 rounding is the same in
+ * every alike bulk cell and so adds up.  Set it before zsc_adj_scratch_bytes
+ * is read; ZSC_ERR_PRECISION for other combinations. */
+int zsc_adj_set_lambda_precision(zsc_ctx *ctx, int precision);
 /* a0 = init_f/init_g equilibria of phi0 (src/init_f.cu, src/init_g.cu). */
 int zsc_adj_init(zsc_ctx *ctx, const void *phi0, void *a0);
 /* a_out = one time step of a_in (update, collide, stream). */

@@ -101,6 +101,7 @@ SIGNATURES = {
     "zsc_bytes_per_cell": (_D, [_I]),
     "zsc_copy_ceiling": (_I, [_VP, _DP]),
     "zsc_adj_scratch_bytes": (ctypes.c_size_t, [_VP]),
+    "zsc_adj_set_lambda_precision": (_I, [_VP, _I]),
     "zsc_adj_init": (_I, [_VP, _VP, _VP]),
     "zsc_adj_forward": (_I, [_VP, _VP, _VP, _VP]),
     "zsc_adj_seed": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),

@@ -80,13 +80,19 @@ class AdjointRunner:
 
     def __init__(self, dims, base: zsc.Params, precision: str = "f32", checkpoint_every: int = 20,
                  gravity_on: bool = True, device: str = "cuda", timing: bool = False,
-                 tape="auto", tape_fraction: float = 0.5):
-        """tape: True keeps every step's input state and phi (no recomputation),
-        False a checkpoint every ``checkpoint_every`` steps, "auto" the tape
-        when it takes at most ``tape_fraction`` of the device memory that is
-        free (to the driver or in torch's cache) when the forward pass starts."""
-        if precision not in ("f32", "f64"):
-            raise ValueError("the adjoint runs at f32 or f64")
+                 tape="auto", tape_fraction: float = 0.5, states: str | None = None):
+        """precision: storage of the adjoint lattices; states: precision of
+        the forward run and its stored states (default: the same).  FP32
+        adjoints over FP64 states (precision="f32", states="f64") keep the
+        FP64 forward and FP64 per-cell reverse arithmetic at FP32 adjoint
+        storage.  tape: True keeps every step's input state and phi (no
+        recomputation), False a checkpoint every ``checkpoint_every`` steps,
+        "auto" the tape when it takes at most ``tape_fraction`` of the device
+        memory that is free (to the driver or in torch's cache) when the
+        forward pass starts."""
+        states = states or precision
+        if (precision, states) not in (("f32", "f32"), ("f64", "f64"), ("f32", "f64")):
+            raise ValueError("the adjoint runs at f32 or f64, or f32 over f64 states")
         if tape not in (True, False, "auto"):
             raise ValueError("tape is True, False or 'auto'")
         if int(checkpoint_every) < 1:
@@ -99,13 +105,15 @@ class AdjointRunner:
         self.n = nx * ny * nz
         self.base = base
         self.k = int(checkpoint_every)
-        self.dtype = torch.float32 if precision == "f32" else torch.float64
+        self.dtype = torch.float32 if states == "f32" else torch.float64    # states, phi
+        self.ldtype = torch.float32 if precision == "f32" else torch.float64  # adjoint lattices
         self.device = device
         # A stream of our own (default priority): the library's kernels and
         # the torch ops on these buffers are ordered on it.  torch's default
         # stream has handle 0, which the C-ABI reads as "make a new stream".
         self.stream = torch.cuda.Stream(device=device)
-        self.ctx = zsc.Context(nx, ny, nz, precision=precision, stream=self.stream.cuda_stream)
+        self.ctx = zsc.Context(nx, ny, nz, precision=states, stream=self.stream.cuda_stream)
+        check(lib().zsc_adj_set_lambda_precision(self.ctx._h, zsc._lib.PRECISIONS[precision]))
         self.ctx.set_gravity(gravity_on)
         nbytes = lib().zsc_adj_scratch_bytes(self.ctx._h)
         self.scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
@@ -120,6 +128,9 @@ class AdjointRunner:
 
     def _lattice(self):
         return torch.empty((26, self.n), dtype=self.dtype, device=self.device)
+
+    def _lam(self):
+        return torch.empty((26, self.n), dtype=self.ldtype, device=self.device)
 
     def _set_theta(self, theta):
         self.theta = [float(t) for t in theta]
@@ -202,7 +213,7 @@ class AdjointRunner:
     def _loss(self, a_n, phi_tgt, lam):
         self.grads.zero_()
         tgt = phi_tgt.to(self.device, self.dtype).contiguous().view(-1)
-        lam = self._lattice() if lam is None else lam
+        lam = self._lam() if lam is None else lam
         check(lib().zsc_adj_seed(self.ctx._h, _vp(a_n), _vp(tgt), _vp(lam), _vp(self.scratch),
                                  ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))))
         return float(self.grads[14].item()), lam
@@ -246,7 +257,7 @@ class AdjointRunner:
         gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
         loss_slot = float(self.grads[14].item())
         self.grads.zero_()
-        lam2 = self._lattice()
+        lam2 = self._lam()
         if self.tape is not None:
             # every step's input is on the tape: one segment, nothing to recompute
             segments = [(0, self.nsteps, self.tape)]

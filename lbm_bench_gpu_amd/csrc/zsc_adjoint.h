@@ -270,12 +270,14 @@ __device__ inline int64_t adj_first(int remap)
 
 // Pass 1: lam_out (local part, including phi's local uses), the stencil-
 // adjoint fields a_lap, a_grad (4 fields) and per-block coefficient
-// partials.  A and lam are plain (26, n) lattices.
-template <class R, bool PULL>
+// partials.  A and lam are plain (26, n) lattices.  The state A, phi and all
+// arithmetic are at precision R; the adjoint lattices and the fields are
+// stored at precision L (L = R, or FP32 adjoints over FP64 states).
+template <class R, class L, bool PULL>
 __global__ void __launch_bounds__(256)
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
-                const R *__restrict__ phi, const R *__restrict__ lam, R *__restrict__ lam_out,
-                R *__restrict__ fields, double *partial, int accumulate, int remap)
+                const R *__restrict__ phi, const L *__restrict__ lam, L *__restrict__ lam_out,
+                L *__restrict__ fields, double *partial, int accumulate, int remap)
 {
     R ap[kNumGrad];
 #pragma unroll
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(256)
         lfd[0] = R(0);
 #pragma unroll
         for (int i = 1; i < 7; ++i) {
-            const R *li = lam + (int64_t)i * n;
+            const L *li = lam + (int64_t)i * n;
             const R mine = ld32z<0>(li, own);
             lfd[i] = ld32z<0>(li, nb.at(cx(i), cy(i), cz(i)));
             lf[i] = c.eta * lfd[i] + c.eta2 * mine;
@@ -321,13 +323,13 @@ __global__ void __launch_bounds__(256)
         adj_cell(c, p, fa, ga, lf, lfd, lg, afa, aga, aphi, alap, agr, deta, ap, grav_on);
         ap[kGEta] += deta;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, own, afa[i] + aphi);
+        for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, own, (L)(afa[i] + aphi));
 #pragma unroll
-        for (int i = 0; i < 19; ++i) st32z<0>(lam_out + (int64_t)(7 + i) * n, own, aga[i]);
-        st32z<0>(fields, own, alap);
-        st32z<0>(fields + n, own, agr[0]);
-        st32z<0>(fields + 2 * n, own, agr[1]);
-        st32z<0>(fields + 3 * n, own, agr[2]);
+        for (int i = 0; i < 19; ++i) st32z<0>(lam_out + (int64_t)(7 + i) * n, own, (L)aga[i]);
+        st32z<0>(fields, own, (L)alap);
+        st32z<0>(fields + n, own, (L)agr[0]);
+        st32z<0>(fields + 2 * n, own, (L)agr[1]);
+        st32z<0>(fields + 3 * n, own, (L)agr[2]);
     }
     double apd[kNumGrad];
 #pragma unroll
@@ -438,9 +440,9 @@ __global__ void __launch_bounds__(64 * TY)
 }
 
 // Loss seed: L = sum (phi - phi_tgt)^2, lam_f,i = 2 (phi - phi_tgt), lam_g = 0.
-template <class R>
+template <class R, class L>
 __global__ void __launch_bounds__(256)
-    k_adj_seed(int64_t n, const R *__restrict__ A, const R *__restrict__ tgt, R *__restrict__ lam,
+    k_adj_seed(int64_t n, const R *__restrict__ A, const R *__restrict__ tgt, L *__restrict__ lam,
                double *partial)
 {
     double ap[kNumGrad];
@@ -454,19 +456,19 @@ __global__ void __launch_bounds__(256)
         const R d = sum_f(f) - tgt[idx];
         ap[kGLoss] += (double)d * (double)d;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) lam[(int64_t)i * n + idx] = R(2) * d;
+        for (int i = 0; i < 7; ++i) lam[(int64_t)i * n + idx] = (L)(R(2) * d);
 #pragma unroll
-        for (int i = 0; i < 19; ++i) lam[(int64_t)(7 + i) * n + idx] = R(0);
+        for (int i = 0; i < 19; ++i) lam[(int64_t)(7 + i) * n + idx] = L(0);
     }
     block_reduce_grads<256>(ap, partial);
 }
 
 // Adjoint of the initial equilibria (src/init_f.cu:69-78, src/init_g.cu:74-91)
 // with respect to alpha4 and kappa, at fixed phi0 and rho = rho_mean.
-template <class R>
+template <class R, class L>
 __global__ void __launch_bounds__(256)
     k_adj_init(int nx, int ny, int nz, int64_t n, Coef<R> c, const R *__restrict__ phi0,
-               const R *__restrict__ lam, double *partial)
+               const L *__restrict__ lam, double *partial)
 {
     double ap[kNumGrad];
 #pragma unroll
@@ -479,14 +481,15 @@ __global__ void __launch_bounds__(256)
         R p[19];
         gather19(phi0, nb, p);
         const R phi = p[0];
-        R a_mu = R(-3) * c.gamma * lam[idx];
+        auto lm = [&](int64_t i) { return (R)lam[i]; };
+        R a_mu = R(-3) * c.gamma * lm(idx);
 #pragma unroll
-        for (int i = 1; i < 7; ++i) a_mu += R(0.5) * c.gamma * lam[(int64_t)i * n + idx];
-        a_mu += R(-6) * W<R>::w0 * phi * lam[7 * n + idx];
+        for (int i = 1; i < 7; ++i) a_mu += R(0.5) * c.gamma * lm((int64_t)i * n + idx);
+        a_mu += R(-6) * W<R>::w0 * phi * lm(7 * n + idx);
 #pragma unroll
-        for (int i = 1; i < 7; ++i) a_mu += R(3) * W<R>::w1 * phi * lam[(int64_t)(7 + i) * n + idx];
+        for (int i = 1; i < 7; ++i) a_mu += R(3) * W<R>::w1 * phi * lm((int64_t)(7 + i) * n + idx);
 #pragma unroll
-        for (int i = 7; i < 19; ++i) a_mu += R(3) * W<R>::w2 * phi * lam[(int64_t)(7 + i) * n + idx];
+        for (int i = 7; i < 19; ++i) a_mu += R(3) * W<R>::w2 * phi * lm((int64_t)(7 + i) * n + idx);
         ap[kGAlpha4] += (double)(phi * (phi * phi - c.phi_star_sq) * a_mu);
         ap[kGKappa] += (double)(-lap19(p) * a_mu);
     }
@@ -523,10 +526,12 @@ __global__ void __launch_bounds__(256) k_adj_accumulate(int nblocks, const doubl
     }
 }
 
-template <class R>
+// R: precision of the states, phi and the arithmetic; L: storage of the
+// adjoint lattices and stencil fields (R, or float over double states).
+template <class R, class L>
 struct Adj {
-    // scratch: 4n R stencil fields | partials | deferred partials
-    static size_t field_bytes(int64_t n) { return align256((size_t)n * 4 * sizeof(R)); }
+    // scratch: 4n L stencil fields | partials | deferred partials
+    static size_t field_bytes(int64_t n) { return align256((size_t)n * 4 * sizeof(L)); }
     static size_t scratch_bytes(int64_t n)
     {
         return field_bytes(n) + 2 * (size_t)kAdjMaxBlocks * kNumGrad * sizeof(double);
@@ -541,22 +546,24 @@ struct Adj {
     // kept in the scratch buffer until flush(); defer_first: no deferred
     // sums exist yet, so the partials are written rather than added to.
     static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
-                               const R *A, bool pulled, const R *phi, const R *lam, R *lam_out,
+                               const R *A, bool pulled, const R *phi, const L *lam, L *lam_out,
                                void *scratch, double *grads, bool defer_first)
     {
         if ((uint64_t)g.n * sizeof(R) > ((uint64_t)1 << 32)) return hipErrorInvalidValue;
-        R *fields = (R *)scratch;
+        L *fields = (L *)scratch;
         const int acc = grads || defer_first ? 0 : 1;
         const int nb = adj_blocks(g.n);
         double *partial = partials(g, scratch, grads ? 0 : 1);
         const Coef<R> c = make_coef<R>(p, grav_on);
         const int remap = 1;
         if (pulled)
-            hipLaunchKernelGGL((k_adj_pass1<R, true>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
-                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
+            hipLaunchKernelGGL((k_adj_pass1<R, L, true>), dim3(nb), dim3(256), 0, s, g.nx, g.ny,
+                               g.nz, g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc,
+                               remap);
         else
-            hipLaunchKernelGGL((k_adj_pass1<R, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz,
-                               g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
+            hipLaunchKernelGGL((k_adj_pass1<R, L, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny,
+                               g.nz, g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc,
+                               remap);
         {
             constexpr int TY = 4;
             const int nbx = (g.nx + 63) / 64, nby = (g.ny + TY - 1) / TY;
@@ -565,9 +572,9 @@ struct Adj {
                 zchunk /= 2;
             const int64_t nblk = (int64_t)nbx * nby * ((g.nz + zchunk - 1) / zchunk);
             if (nblk > 0x7fffffff) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_adj_pass2<R, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
+            hipLaunchKernelGGL((k_adj_pass2<L, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
                                g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk, remap,
-                               (const R *)fields, lam_out);
+                               (const L *)fields, lam_out);
         }
         if (grads)
             hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
@@ -580,21 +587,21 @@ struct Adj {
                            partials(g, scratch, 1), grads);
         return hipGetLastError();
     }
-    static hipError_t seed(const Geo &g, hipStream_t s, const R *A, const R *tgt, R *lam,
+    static hipError_t seed(const Geo &g, hipStream_t s, const R *A, const R *tgt, L *lam,
                            void *scratch, double *grads)
     {
         double *partial = partials(g, scratch, 0);
         const int nb = adj_blocks(g.n);
-        hipLaunchKernelGGL(k_adj_seed<R>, dim3(nb), dim3(256), 0, s, g.n, A, tgt, lam, partial);
+        hipLaunchKernelGGL((k_adj_seed<R, L>), dim3(nb), dim3(256), 0, s, g.n, A, tgt, lam, partial);
         hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();
     }
     static hipError_t init_backward(const Geo &g, const zsc_params &p, hipStream_t s, const R *phi0,
-                                    const R *lam, void *scratch, double *grads)
+                                    const L *lam, void *scratch, double *grads)
     {
         double *partial = partials(g, scratch, 0);
         const int nb = adj_blocks(g.n);
-        hipLaunchKernelGGL(k_adj_init<R>, dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
+        hipLaunchKernelGGL((k_adj_init<R, L>), dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n,
                            make_coef<R>(p, 0), phi0, lam, partial);
         hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);
         return hipGetLastError();

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -92,6 +93,8 @@ struct zsc_ctx {
     FusedCfg cfg{4, 16, 0};
     // deferred coefficient sums (backward calls with grads == NULL) exist
     bool adj_deferred = false;
+    // FP32 adjoint lattices over the FP64 states of an F64 context
+    bool adj_lam_f32 = false;
 
     void *w(int i) const { return (char *)work + (size_t)i * align256((size_t)geo.n * rsize); }
 };
@@ -1282,10 +1285,29 @@ extern "C" size_t zsc_adj_scratch_bytes(zsc_ctx *c)
     const size_t n = (size_t)c->geo.n;
     // forward: B lattice (26n) + phi (n); backward: phi (n) + Adj scratch
     const size_t fwd = align256(26 * n * c->rsize) + align256(n * c->rsize);
-    const size_t bwd = align256(n * c->rsize) +
-                       (c->prec == ZSC_F64 ? Adj<double>::scratch_bytes(c->geo.n)
-                                           : Adj<float>::scratch_bytes(c->geo.n));
+    const size_t adj = c->prec != ZSC_F64 ? Adj<float>::scratch_bytes(c->geo.n)
+                       : c->adj_lam_f32   ? Adj<double, float>::scratch_bytes(c->geo.n)
+                                          : Adj<double>::scratch_bytes(c->geo.n);
+    const size_t bwd = align256(n * c->rsize) + adj;
     return fwd > bwd ? fwd : bwd;
+}
+
+extern "C" int zsc_adj_set_lambda_precision(zsc_ctx *c, int precision)
+{
+    if (int r = need_ctx(c)) return r;
+    if (c->adj_deferred)
+        return fail(ZSC_ERR_STATE, "zsc_adj_set_lambda_precision: deferred coefficient sums are kept");
+    if (precision == c->prec && (c->prec == ZSC_F64 || c->prec == ZSC_F32)) {
+        c->adj_lam_f32 = false;
+        return ZSC_OK;
+    }
+    if (c->prec == ZSC_F64 && precision == ZSC_F32) {
+        c->adj_lam_f32 = true;
+        return ZSC_OK;
+    }
+    return fail(ZSC_ERR_PRECISION, "zsc_adj_set_lambda_precision: adjoint lattices at precision %d "
+                                   "over a context at %d (F64 or F32 as the context, or F32 over F64)",
+                precision, c->prec);
 }
 
 template <class F>
@@ -1294,6 +1316,19 @@ static int adj_call(zsc_ctx *c, const char *name, F &&f)
     if (int r = need_ctx(c)) return r;
     if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
     return dropin(c, false, name, std::forward<F>(f));
+}
+
+// adj_call for the entry points that take adjoint lattices: f(state tag,
+// adjoint-lattice tag).
+template <class F>
+static int adj_lam_call(zsc_ctx *c, const char *name, F &&f)
+{
+    return adj_call(c, name, [&](auto tag) -> hipError_t {
+        using T = ZSC_T;
+        if constexpr (std::is_same<T, double>::value)
+            if (c->adj_lam_f32) return f(tag, Tag<float>{});
+        return f(tag, tag);
+    });
 }
 
 extern "C" int zsc_adj_init(zsc_ctx *c, const void *phi0, void *a0)
@@ -1329,12 +1364,13 @@ extern "C" int zsc_adj_seed(zsc_ctx *c, const void *a_n, const void *phi_tgt, vo
                             void *scratch, double *grads)
 {
     if (!a_n || !phi_tgt || !lam || !scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_seed: null pointer");
-    return adj_call(c, "zsc_adj_seed", [&](auto tag) {
+    return adj_lam_call(c, "zsc_adj_seed", [&](auto tag, auto ltag) {
         using T = ZSC_T;
+        using L = typename decltype(ltag)::type;
         void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
         c->adj_deferred = false;
-        return Adj<T>::seed(c->geo, c->stream, (const T *)a_n, (const T *)phi_tgt, (T *)lam, adj_scr,
-                            grads);
+        return Adj<T, L>::seed(c->geo, c->stream, (const T *)a_n, (const T *)phi_tgt, (L *)lam,
+                               adj_scr, grads);
     });
 }
 
@@ -1343,8 +1379,9 @@ extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const
 {
     if (!state || !lam_next || !lam_out || !scratch || lam_next == lam_out || (pulled && !phi_n))
         return fail(ZSC_ERR_ARG, "zsc_adj_backward: null or aliased pointer");
-    return adj_call(c, "zsc_adj_backward", [&](auto tag) {
+    return adj_lam_call(c, "zsc_adj_backward", [&](auto tag, auto ltag) {
         using T = ZSC_T;
+        using L = typename decltype(ltag)::type;
         void *phi = scratch;
         void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
         if (!phi_n) {
@@ -1352,9 +1389,10 @@ extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const
             if (e != hipSuccess) return e;
             phi_n = phi;
         }
-        hipError_t e = Adj<T>::backward(c->geo, c->params, c->grav_on, c->stream, (const T *)state,
-                                        pulled != 0, (const T *)phi_n, (const T *)lam_next,
-                                        (T *)lam_out, adj_scr, grads, !grads && !c->adj_deferred);
+        hipError_t e = Adj<T, L>::backward(c->geo, c->params, c->grav_on, c->stream,
+                                           (const T *)state, pulled != 0, (const T *)phi_n,
+                                           (const L *)lam_next, (L *)lam_out, adj_scr, grads,
+                                           !grads && !c->adj_deferred);
         if (e == hipSuccess && !grads) c->adj_deferred = true;
         return e;
     });
@@ -1364,10 +1402,12 @@ extern "C" int zsc_adj_flush(zsc_ctx *c, void *scratch, double *grads)
 {
     if (!scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_flush: null pointer");
     if (c && !c->adj_deferred) return ZSC_OK;
-    return adj_call(c, "zsc_adj_flush", [&](auto tag) {
+    return adj_lam_call(c, "zsc_adj_flush", [&](auto tag, auto ltag) {
         using T = ZSC_T;
-        hipError_t e = Adj<T>::flush(c->geo, c->stream,
-                                     (char *)scratch + align256((size_t)c->geo.n * sizeof(T)), grads);
+        using L = typename decltype(ltag)::type;
+        hipError_t e = Adj<T, L>::flush(c->geo, c->stream,
+                                        (char *)scratch + align256((size_t)c->geo.n * sizeof(T)),
+                                        grads);
         if (e == hipSuccess) c->adj_deferred = false;
         return e;
     });
@@ -1406,10 +1446,11 @@ extern "C" int zsc_adj_init_backward(zsc_ctx *c, const void *phi0, const void *l
                                      double *grads)
 {
     if (!phi0 || !lam0 || !scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_init_backward: null pointer");
-    return adj_call(c, "zsc_adj_init_backward", [&](auto tag) {
+    return adj_lam_call(c, "zsc_adj_init_backward", [&](auto tag, auto ltag) {
         using T = ZSC_T;
+        using L = typename decltype(ltag)::type;
         void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
-        return Adj<T>::init_backward(c->geo, c->params, c->stream, (const T *)phi0, (const T *)lam0,
-                                     adj_scr, grads);
+        return Adj<T, L>::init_backward(c->geo, c->params, c->stream, (const T *)phi0,
+                                        (const L *)lam0, adj_scr, grads);
     });
 }

@@ -7,4 +7,6 @@ namespace zsc {
 template struct Ops<double>;
 template struct DropIn<double>;
 template struct Adj<double>;
+// FP32 adjoint lattices over FP64 states (arithmetic in FP64)
+template struct Adj<double, float>;
 }  // namespace zsc

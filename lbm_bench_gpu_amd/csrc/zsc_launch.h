@@ -121,8 +121,9 @@ struct DropIn {
                                    T *const g[19]);
 };
 
-// Reverse mode of the step on plain (26, n) lattices (zsc_adjoint.h).
-template <class R>
+// Reverse mode of the step on plain (26, n) lattices (zsc_adjoint.h): states
+// and arithmetic at R, adjoint lattices stored at L.
+template <class R, class L = R>
 struct Adj;
 
 }  // namespace zsc

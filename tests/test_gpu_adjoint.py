@@ -21,6 +21,13 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _runner_kw(precision):
+    """AdjointRunner arguments of a test precision: "f64", "f32", or
+    "f32/f64" (FP32 adjoint lattices over FP64 states)."""
+    lam, _, states = precision.partition("/")
+    return {"precision": lam, "states": states or lam}
+
+
 def small_problem(dims, seed=0):
     nx, ny, nz = dims
     p = zsc.derive_params(configs.physics(*dims, gravity=1e-3), 4.0)
@@ -91,7 +98,7 @@ def test_fp64_gradients_match_taped_cpu(need_gpu, tape):
         assert abs(a - r) <= 1e-8 * max(abs(r), 1e-30), (a, r)
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64"])
 def test_tape_equals_recomputation(need_gpu, precision):
     """The tape holds the same states and phi_n that the checkpointed
     backward pass recomputes (the same kernels on the same inputs), so both
@@ -104,7 +111,7 @@ def test_tape_equals_recomputation(need_gpu, precision):
     tt = [theta[0] * 1.1, theta[1] * 0.95, theta[2], theta[3], theta[4] * 1.3]
     out = {}
     for tape in (True, False):
-        run = adjoint.AdjointRunner(dims, p, precision=precision, checkpoint_every=4, tape=tape)
+        run = adjoint.AdjointRunner(dims, p, checkpoint_every=4, tape=tape, **_runner_kw(precision))
         tgt = run.forward(tt, torch.from_numpy(phi0), 11, keep=False)[:7].sum(0).clone()
         L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, 11)
         assert (run.tape is not None) == tape
@@ -126,8 +133,8 @@ def _workload_fd(precision, gravity_on):
     fd_run = adjoint.AdjointRunner(w.dims, p, precision="f64", checkpoint_every=20,
                                    gravity_on=gravity_on)
     tgt = fd_run.forward(tt, phi0, w.steps, keep=False)[:7].sum(0).clone()
-    run = adjoint.AdjointRunner(w.dims, p, precision=precision, checkpoint_every=20,
-                                gravity_on=gravity_on)
+    run = adjoint.AdjointRunner(w.dims, p, checkpoint_every=20, gravity_on=gravity_on,
+                                **_runner_kw(precision))
     L, grads = run.loss_and_grad(theta, phi0, tgt, w.steps)
     fd = []
     for i in range(5):
@@ -143,10 +150,11 @@ def _workload_fd(precision, gravity_on):
 
 
 SMOOTH_TOL = {"f64": {"kappa": 1e-6, "beta": 1e-6, "tau_f": 1e-6, "tau_g": 1e-6},
+              "f32/f64": {"kappa": 1e-4, "beta": 1e-4, "tau_f": 1e-4, "tau_g": 1e-4},
               "f32": {"kappa": 1e-3, "beta": 1e-3, "tau_f": 2e-2, "tau_g": 5e-2}}
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32/f64", "f32"])
 def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     """The 128^3 workload (8 bubbles R = 12, seed 4, 400 steps, target from
     kappa x1.2, beta x0.9) with gravity switched off, so the loss is smooth
@@ -159,8 +167,10 @@ def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     their distance from FP64 is the FP32 trajectory's; tau_g's derivative,
     1.13 against O(1e4 .. 1e5) for the others, sums lambda (g - g_eq), whose
     non-equilibrium part FP32 states (g stored absolutely) carry to ~1e-3
-    per cell: ~1.5% off.  The bounds are those measurements with room; the
-    gravity derivative is 0."""
+    per cell: ~1.5% off.  FP32 adjoint lattices over FP64 states ("f32/f64":
+    FP64 forward and per-cell reverse arithmetic, FP32 lambda storage) meet
+    1e-4 for every parameter.  The FP32 bounds are those measurements with
+    room; the gravity derivative is 0."""
     grads, fd = _workload_fd(precision, False)
     errs = {name: abs(a - r) / abs(r) for name, a, r in list(zip(adjoint.THETA, grads, fd))[:4]}
     print(precision, "relative error vs FD", errs, flush=True)
@@ -168,7 +178,7 @@ def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     assert grads[4] == 0.0
 
 
-@pytest.mark.parametrize("precision,tol", [("f64", 5e-4), ("f32", 3e-3)])
+@pytest.mark.parametrize("precision,tol", [("f64", 5e-4), ("f32/f64", 5e-4), ("f32", 3e-3)])
 def test_adjoint_workload_vs_central_differences(need_gpu, precision, tol):
     """The 128^3 workload as specified (gravity x1.5 in the target).  The
     phi > 0 gravity switch makes the loss piecewise smooth: central
@@ -181,7 +191,7 @@ def test_adjoint_workload_vs_central_differences(need_gpu, precision, tol):
         assert abs(a - r) <= tol * abs(r), (name, a, r)
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64"])
 def test_deferred_coefficient_sums_equal_immediate(need_gpu, precision):
     """Backward calls with grads = NULL keep the coefficient sums in the
     scratch buffer, also when it is fresh (no zsc_adj_seed before); one
@@ -191,7 +201,7 @@ def test_deferred_coefficient_sums_equal_immediate(need_gpu, precision):
     from lbm_bench_gpu_amd._lib import ZSC_ERR_STATE, ZscError
     dims = (14, 10, 8)
     p, phi0 = small_problem(dims, seed=3)
-    run = adjoint.AdjointRunner(dims, p, precision=precision, checkpoint_every=4)
+    run = adjoint.AdjointRunner(dims, p, checkpoint_every=4, **_runner_kw(precision))
     h = run.ctx._h
     a = [run.forward(adjoint.theta_of(p), torch.from_numpy(phi0), 2)]
     for _ in range(3):
@@ -201,13 +211,13 @@ def test_deferred_coefficient_sums_equal_immediate(need_gpu, precision):
         a.append(nxt)
     run.ctx.sync()
     g = torch.Generator(device="cuda").manual_seed(5)
-    lam3 = torch.randn((26, run.n), dtype=run.dtype, device="cuda", generator=g)
+    lam3 = torch.randn((26, run.n), dtype=run.ldtype, device="cuda", generator=g)
     run.scratch.fill_(0x7f)  # garbage where the deferred sums will live
     gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
     out = {}
     for mode in ("deferred", "immediate"):
         run.grads.zero_()
-        lam, lam2 = lam3.clone(), run._lattice()
+        lam, lam2 = lam3.clone(), run._lam()
         torch.cuda.synchronize()  # torch ops above ran on torch's stream
         for n in (2, 1, 0):
             check(lib().zsc_adj_backward(h, _vp(a[n]), 0, None, _vp(lam), _vp(lam2),
@@ -263,3 +273,39 @@ def test_autograd_function_gives_the_runner_gradients(need_gpu):
     assert Lv == L_ref
     for a, r in zip(theta.grad.tolist(), g_ref):
         assert a == pytest.approx(3.0 * r, rel=1e-14, abs=0.0)
+
+
+def test_mixed_adjoint_dot_product_and_refusals(need_gpu):
+    """FP32 adjoint lattices over FP64 states: one step's <J v, w> (FP64
+    central differences) against <v, J^T w> with w stored in FP32 (relative
+    1e-6: the FP32 rounding of the stored adjoint), and the lambda precision
+    combinations the library refuses."""
+    import torch
+    from lbm_bench_gpu_amd._lib import ZSC_ERR_PRECISION, PRECISIONS
+    dims = (12, 10, 8)
+    p, phi0 = small_problem(dims)
+    run = adjoint.AdjointRunner(dims, p, precision="f32", states="f64", checkpoint_every=4)
+    a = run.forward(adjoint.theta_of(p), torch.from_numpy(phi0), 3)
+    n = run.n
+    g = torch.Generator(device="cuda").manual_seed(1)
+    v = torch.randn((26, n), dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn((26, n), dtype=torch.float64, device="cuda", generator=g).float()
+    eps = 1e-6
+    ap, am = a + eps * v, a - eps * v
+    op, om = run._lattice(), run._lattice()
+    lam = run._lam()
+    torch.cuda.synchronize()
+    check(lib().zsc_adj_forward(run.ctx._h, _vp(ap), _vp(op), _vp(run.scratch)))
+    check(lib().zsc_adj_forward(run.ctx._h, _vp(am), _vp(om), _vp(run.scratch)))
+    gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
+    check(lib().zsc_adj_backward(run.ctx._h, _vp(a), 0, None, _vp(w), _vp(lam), _vp(run.scratch),
+                                 gp))
+    run.ctx.sync()
+    jv = (op - om) / (2 * eps)
+    lhs = float((jv * w.double()).sum())
+    rhs = float((v * lam.double()).sum())
+    print("mixed dot product", lhs, rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-6 * abs(rhs)
+    for ctx_prec, lam_prec in (("f32", "f64"), ("f32", "bf16"), ("f64", "bf16")):
+        with zsc.Context(*dims, precision=ctx_prec) as ctx:
+            assert lib().zsc_adj_set_lambda_precision(ctx._h, PRECISIONS[lam_prec]) == ZSC_ERR_PRECISION
