@@ -54,155 +54,174 @@ enum : int {
 // eta derivative of the streaming, fb the forward post-collision f).  Each
 // intermediate is formed as late, and dropped as early, as the data flow
 // allows, which keeps the kernel's register count down.
-template <class R>
-__device__ inline void adj_cell(const Coef<R> &c, const R *p, const R *fa, const R *ga,
-                                const R *lf, const R *lfd, const R *lg, R *afa, R *aga, R &aphi,
-                                R &alap, R *agr, R &deta, R *ap, int grav_on)
+//
+// P is the precision of the states, the forward recompute and the
+// coefficient derivatives ap (with c); R that of the adjoint quantities
+// (with cr, the same coefficients at R).  With P = R every expression is the
+// plain one at that precision.  FP32 adjoints over FP64 states (P = double,
+// R = float) keep in FP64 what cancels: the coefficient sums pair large
+// primal values (sum_i lam_i g_i against lambda-weighted equilibria) whose
+// difference is ~1e-4 of either.
+template <class P, class R>
+__device__ inline void adj_cell(const Coef<P> &c, const Coef<R> &cr, const P *p, const P *fa,
+                                const P *ga, const R *lf, const P *lfd, const R *lg, R *afa,
+                                R *aga, R &aphi, R &alap, R *agr, P &deta, P *ap, int grav_on)
 {
     // ---- forward recompute
-    const R phi = p[0];
-    const R rho = sum_g(ga);
-    const R jx = ga[1] - ga[2] + ga[7] - ga[8] + ga[9] - ga[10] + ga[11] - ga[12] + ga[13] - ga[14];
-    const R jy = ga[3] - ga[4] + ga[7] - ga[8] - ga[9] + ga[10] + ga[15] - ga[16] + ga[17] - ga[18];
-    const R jz = ga[5] - ga[6] + ga[11] - ga[12] - ga[13] + ga[14] + ga[15] - ga[16] - ga[17] + ga[18];
+    const P phi = p[0];
+    const P rho = sum_g(ga);
+    const P jx = ga[1] - ga[2] + ga[7] - ga[8] + ga[9] - ga[10] + ga[11] - ga[12] + ga[13] - ga[14];
+    const P jy = ga[3] - ga[4] + ga[7] - ga[8] - ga[9] + ga[10] + ga[15] - ga[16] + ga[17] - ga[18];
+    const P jz = ga[5] - ga[6] + ga[11] - ga[12] - ga[13] + ga[14] + ga[15] - ga[16] - ga[17] + ga[18];
     // quotients by rho: FP32 by the hardware reciprocal, FP64 exact
+    using Qp = ::zsc::Q<P>;
     using Qr = ::zsc::Q<R>;
-    const R u[3] = {Qr::div(jx, rho), Qr::div(jy, rho), Qr::div(jz, rho)};
-    const R lap = lap19(p);
-    const R G[3] = {
-        (R(2) * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] + p[13] - p[14]) *
-            W<R>::inv12,
-        (R(2) * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] + p[17] - p[18]) *
-            W<R>::inv12,
-        (R(2) * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] + p[18] - p[17]) *
-            W<R>::inv12};
-    const R phi3 = phi * (phi * phi - c.phi_star_sq);
-    const R mu = c.alpha4 * phi3 - c.kappa * lap;
-    const R H = phi > R(0) ? R(1) : R(0);
-    R F[3] = {mu * G[0], mu * G[1], mu * G[2] + H * c.grav};
-    R v[3];
-    for (int a = 0; a < 3; ++a) v[a] = u[a] + Qr::div(R(0.5) * F[a], rho);
-    const R hg = R(0.5) * c.gamma * c.inv_tau_phi;
-    const R cfk = c.inv_tau_phi * c.inv_eta2;
-    const R Af = hg * mu, Cf = cfk * phi;
+    const P u[3] = {Qp::div(jx, rho), Qp::div(jy, rho), Qp::div(jz, rho)};
+    const P lap = lap19(p);
+    const P G[3] = {
+        (P(2) * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] + p[13] - p[14]) *
+            W<P>::inv12,
+        (P(2) * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] + p[17] - p[18]) *
+            W<P>::inv12,
+        (P(2) * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] + p[18] - p[17]) *
+            W<P>::inv12};
+    const P phi3 = phi * (phi * phi - c.phi_star_sq);
+    const P mu = c.alpha4 * phi3 - c.kappa * lap;
+    const P H = phi > P(0) ? P(1) : P(0);
+    P F[3] = {mu * G[0], mu * G[1], mu * G[2] + H * c.grav};
+    P v[3];
+    for (int a = 0; a < 3; ++a) v[a] = u[a] + Qp::div(P(0.5) * F[a], rho);
+    const P hg = P(0.5) * c.gamma * c.inv_tau_phi;
+    const P cfk = c.inv_tau_phi * c.inv_eta2;
+    const P Af = hg * mu, Cf = cfk * phi;
     {
-        R fb[7];
-        fb[0] = c.inv_tau_phi_one * fa[0] - R(6) * Af + c.inv_tau_phi * phi;
+        P fb[7];
+        fb[0] = c.inv_tau_phi_one * fa[0] - P(6) * Af + c.inv_tau_phi * phi;
         for (int a = 0; a < 3; ++a) {
             fb[1 + 2 * a] = c.inv_tau_phi_one * fa[1 + 2 * a] + Af + Cf * v[a];
             fb[2 + 2 * a] = c.inv_tau_phi_one * fa[2 + 2 * a] + Af - Cf * v[a];
         }
-        deta = R(0);
+        deta = P(0);
 #pragma unroll
         for (int i = 1; i < 7; ++i) deta += fb[i] * lfd[i];
     }
+    // the primal values the reverse sweep uses, at R
+    const R phi_r = (R)phi, mu_r = (R)mu, rho_r = (R)rho;
 
     // ---- f collision (src/collision_f.cu:105-114), reversed
     R a_phi = R(0), a_mu = R(0), a_rho = R(0), a_v[3] = {0, 0, 0}, a_F[3] = {0, 0, 0};
-    const R tf = c.inv_tau_phi_one;
-    R s_tf = R(0);
-    for (int i = 0; i < 7; ++i) s_tf += fa[i] * lf[i];
-    ap[kGInvTauPhiOne] += (R)s_tf;
+    const R tf = cr.inv_tau_phi_one;
+    P s_tf = P(0);
+    for (int i = 0; i < 7; ++i) s_tf += fa[i] * (P)lf[i];
+    ap[kGInvTauPhiOne] += s_tf;
     R a_Af = R(-6) * lf[0];
     for (int i = 1; i < 7; ++i) a_Af += lf[i];
-    a_phi += c.inv_tau_phi * lf[0];
-    R a_Cf = R(0);
+    a_phi += cr.inv_tau_phi * lf[0];
+    P a_Cf = P(0);
     for (int a = 0; a < 3; ++a) {
-        const R d = lf[1 + 2 * a] - lf[2 + 2 * a];
+        const P d = (P)lf[1 + 2 * a] - (P)lf[2 + 2 * a];
         a_Cf += v[a] * d;
-        a_v[a] += Cf * d;
+        a_v[a] += (R)(Cf * d);
     }
-    a_mu += hg * a_Af;
-    a_phi += cfk * a_Cf;
-    ap[kGInvTauPhi] += (R)(phi * lf[0] + R(0.5) * c.gamma * mu * a_Af + c.inv_eta2 * phi * a_Cf);
-    ap[kGInvEta2] += (R)(c.inv_tau_phi * phi * a_Cf);
+    a_mu += (R)hg * a_Af;
+    a_phi += (R)(cfk * a_Cf);
+    ap[kGInvTauPhi] += phi * (P)lf[0] + P(0.5) * c.gamma * mu * (P)a_Af + c.inv_eta2 * phi * a_Cf;
+    ap[kGInvEta2] += c.inv_tau_phi * phi * a_Cf;
 
-    // ---- g collision (src/collision_g.cu:228-313), reversed
-    const R tg = c.inv_tau_rho_one;
-    R s_tg = R(0);
-    for (int i = 0; i < 19; ++i) s_tg += ga[i] * lg[i];
-    ap[kGInvTauRhoOne] += (R)s_tg;
-    const R ut[3] = {R(3) * v[0], R(3) * v[1], R(3) * v[2]};
-    const R UF = v[0] * F[0] + v[1] * F[1] + v[2] * F[2];
-    const R Q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-    const R X = R(3) * phi * mu + rho * (R(1) - R(1.5) * Q);
+    // ---- g collision (src/collision_g.cu:105-194), reversed
+    const R tg = cr.inv_tau_rho_one;
+    P s_tg = P(0);
+    for (int i = 0; i < 19; ++i) s_tg += ga[i] * (P)lg[i];
+    ap[kGInvTauRhoOne] += s_tg;
+    const P ut[3] = {P(3) * v[0], P(3) * v[1], P(3) * v[2]};
+    const P UF = v[0] * F[0] + v[1] * F[1] + v[2] * F[2];
+    const P Q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const P X = P(3) * phi * mu + rho * (P(1) - P(1.5) * Q);
+    const R Q_r = (R)Q;
+    const R utr[3] = {(R)ut[0], (R)ut[1], (R)ut[2]}, Fr[3] = {(R)F[0], (R)F[1], (R)F[2]};
     R a_UF = R(0), a_Q = R(0), a_X = R(0), a_ut[3] = {0, 0, 0};
-    R g_k0, g_k1 = R(0), g_k2 = R(0), g_kc0, g_kc1 = R(0), g_kc2 = R(0);
+    P g_k0, g_k1 = P(0), g_k2 = P(0), g_kc0, g_kc1 = P(0), g_kc2 = P(0);
     {
         const R D = lg[0];
-        g_k0 = (rho - R(6) * phi * mu - R(1.5) * rho * Q) * D;
-        a_rho += c.k0 * (R(1) - R(1.5) * Q) * D;
-        a_phi += R(-6) * c.k0 * mu * D;
-        a_mu += R(-6) * c.k0 * phi * D;
-        a_Q += R(-1.5) * c.k0 * rho * D;
-        g_kc0 = -UF * D;
-        a_UF += -c.kc0 * D;
+        g_k0 = (rho - P(6) * phi * mu - P(1.5) * rho * Q) * (P)D;
+        a_rho += cr.k0 * (R(1) - R(1.5) * Q_r) * D;
+        a_phi += R(-6) * cr.k0 * mu_r * D;
+        a_mu += R(-6) * cr.k0 * phi_r * D;
+        a_Q += R(-1.5) * cr.k0 * rho_r * D;
+        g_kc0 = -UF * (P)D;
+        a_UF += -cr.kc0 * D;
     }
     for (int a = 0; a < 3; ++a) {
-        const R Sp = lg[1 + 2 * a] + lg[2 + 2 * a], Dl = lg[1 + 2 * a] - lg[2 + 2 * a];
-        const R w = ut[a];
-        a_X += c.k1 * Sp;
-        g_k1 += X * Sp + R(0.5) * rho * w * w * Sp + w * rho * Dl;
-        a_rho += R(0.5) * c.k1 * w * w * Sp + w * c.k1 * Dl;
-        a_ut[a] += (c.k1 * rho * w + c.kc1 * F[a]) * Sp + (c.k1 * rho + c.kc1 * F[a]) * Dl;
-        a_F[a] += c.kc1 * w * Sp + w * c.kc1 * Dl;
+        const P Sp = (P)lg[1 + 2 * a] + (P)lg[2 + 2 * a], Dl = (P)lg[1 + 2 * a] - (P)lg[2 + 2 * a];
+        const R Spr = (R)Sp, Dlr = (R)Dl;
+        const P w = ut[a];
+        const R wr = utr[a];
+        a_X += cr.k1 * Spr;
+        g_k1 += X * Sp + P(0.5) * rho * w * w * Sp + w * rho * Dl;
+        a_rho += R(0.5) * cr.k1 * wr * wr * Spr + wr * cr.k1 * Dlr;
+        a_ut[a] += (cr.k1 * rho_r * wr + cr.kc1 * Fr[a]) * Spr + (cr.k1 * rho_r + cr.kc1 * Fr[a]) * Dlr;
+        a_F[a] += cr.kc1 * wr * Spr + wr * cr.kc1 * Dlr;
         g_kc1 += (w * F[a] - UF) * Sp + w * F[a] * Dl;
-        a_UF += -c.kc1 * Sp;
+        a_UF += -cr.kc1 * Spr;
     }
     const int da[6] = {0, 0, 0, 0, 1, 1}, db[6] = {1, 1, 2, 2, 2, 2};
-    const R sg[6] = {R(1), R(-1), R(1), R(-1), R(1), R(-1)};
+    const P sg[6] = {P(1), P(-1), P(1), P(-1), P(1), P(-1)};
+    const R sgr[6] = {R(1), R(-1), R(1), R(-1), R(1), R(-1)};
     for (int d = 0; d < 6; ++d) {
         const int ip = 7 + 2 * d, im = 8 + 2 * d, a = da[d], b = db[d];
-        const R Sp = lg[ip] + lg[im], Dl = lg[ip] - lg[im];
-        const R s = ut[a] + sg[d] * ut[b], S = F[a] + sg[d] * F[b];
-        a_X += c.k2 * Sp;
-        g_k2 += X * Sp + R(0.5) * rho * s * s * Sp + rho * s * Dl;
-        a_rho += R(0.5) * c.k2 * s * s * Sp + c.k2 * s * Dl;
-        const R a_s = (c.k2 * rho * s + c.kc2 * S) * Sp + c.k2 * rho * Dl;
-        const R a_S = c.kc2 * s * Sp + c.kc2 * Dl;
+        const P Sp = (P)lg[ip] + (P)lg[im], Dl = (P)lg[ip] - (P)lg[im];
+        const R Spr = (R)Sp, Dlr = (R)Dl;
+        const P s = ut[a] + sg[d] * ut[b], S = F[a] + sg[d] * F[b];
+        const R sr = (R)s, Sr = (R)S;
+        a_X += cr.k2 * Spr;
+        g_k2 += X * Sp + P(0.5) * rho * s * s * Sp + rho * s * Dl;
+        a_rho += R(0.5) * cr.k2 * sr * sr * Spr + cr.k2 * sr * Dlr;
+        const R a_s = (cr.k2 * rho_r * sr + cr.kc2 * Sr) * Spr + cr.k2 * rho_r * Dlr;
+        const R a_S = cr.kc2 * sr * Spr + cr.kc2 * Dlr;
         g_kc2 += (s * S - UF) * Sp + S * Dl;
-        a_UF += -c.kc2 * Sp;
+        a_UF += -cr.kc2 * Spr;
         a_ut[a] += a_s;
-        a_ut[b] += sg[d] * a_s;
+        a_ut[b] += sgr[d] * a_s;
         a_F[a] += a_S;
-        a_F[b] += sg[d] * a_S;
+        a_F[b] += sgr[d] * a_S;
     }
-    ap[kGK0] += (R)g_k0;
-    ap[kGK1] += (R)g_k1;
-    ap[kGK2] += (R)g_k2;
-    ap[kGKC0] += (R)g_kc0;
-    ap[kGKC1] += (R)g_kc1;
-    ap[kGKC2] += (R)g_kc2;
+    ap[kGK0] += g_k0;
+    ap[kGK1] += g_k1;
+    ap[kGK2] += g_k2;
+    ap[kGKC0] += g_kc0;
+    ap[kGKC1] += g_kc1;
+    ap[kGKC2] += g_kc2;
     // X = 3 phi mu + rho (1 - 1.5 Q)
-    a_phi += R(3) * mu * a_X;
-    a_mu += R(3) * phi * a_X;
-    a_rho += (R(1) - R(1.5) * Q) * a_X;
-    a_Q += R(-1.5) * rho * a_X;
+    a_phi += R(3) * mu_r * a_X;
+    a_mu += R(3) * phi_r * a_X;
+    a_rho += (R(1) - R(1.5) * Q_r) * a_X;
+    a_Q += R(-1.5) * rho_r * a_X;
+    const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
     for (int a = 0; a < 3; ++a) {
-        a_v[a] += R(3) * a_ut[a] + R(2) * v[a] * a_Q + F[a] * a_UF;
-        a_F[a] += v[a] * a_UF;
+        a_v[a] += R(3) * a_ut[a] + R(2) * vr[a] * a_Q + Fr[a] * a_UF;
+        a_F[a] += vr[a] * a_UF;
     }
     // v = u + F / (2 rho)
     R a_u[3];
     for (int a = 0; a < 3; ++a) {
         a_u[a] = a_v[a];
-        a_F[a] += Qr::div(a_v[a] * R(0.5), rho);
-        a_rho += Qr::div(-F[a] * a_v[a] * R(0.5), rho * rho);
+        a_F[a] += Qr::div(a_v[a] * R(0.5), rho_r);
+        a_rho += Qr::div(-Fr[a] * a_v[a] * R(0.5), rho_r * rho_r);
     }
     // F = mu grad(phi) + grav H(phi) e_z
-    a_mu += G[0] * a_F[0] + G[1] * a_F[1] + G[2] * a_F[2];
-    for (int a = 0; a < 3; ++a) agr[a] = mu * a_F[a];
-    if (grav_on) ap[kGGrav] += (R)(H * a_F[2]);
+    a_mu += (R)G[0] * a_F[0] + (R)G[1] * a_F[1] + (R)G[2] * a_F[2];
+    for (int a = 0; a < 3; ++a) agr[a] = mu_r * a_F[a];
+    if (grav_on) ap[kGGrav] += H * (P)a_F[2];
     // mu = alpha4 phi (phi^2 - phi*^2) - kappa lap
-    a_phi += c.alpha4 * (R(3) * phi * phi - c.phi_star_sq) * a_mu;
-    ap[kGAlpha4] += (R)(phi3 * a_mu);
-    ap[kGKappa] += (R)(-lap * a_mu);
-    alap = -c.kappa * a_mu;
+    a_phi += cr.alpha4 * (R(3) * phi_r * phi_r - cr.phi_star_sq) * a_mu;
+    ap[kGAlpha4] += phi3 * (P)a_mu;
+    ap[kGKappa] += -lap * (P)a_mu;
+    alap = -cr.kappa * a_mu;
     // u = j / rho, rho = sum g, j = sum c g
     R a_j[3];
     for (int a = 0; a < 3; ++a) {
-        a_j[a] = Qr::div(a_u[a], rho);
-        a_rho += Qr::div(-u[a] * a_u[a], rho);
+        a_j[a] = Qr::div(a_u[a], rho_r);
+        a_rho += Qr::div(-(R)u[a] * a_u[a], rho_r);
     }
 #pragma unroll
     for (int i = 0; i < 7; ++i) afa[i] = tf * lf[i];
@@ -273,11 +292,20 @@ __device__ inline int64_t adj_first(int remap)
 // partials.  A and lam are plain (26, n) lattices.  The state A, phi and all
 // arithmetic are at precision R; the adjoint lattices and the fields are
 // stored at precision L (L = R, or FP32 adjoints over FP64 states).
+// FP64 states need more registers than the file holds at two waves per SIMD;
+// the bound keeps two resident (spilling what does not fit) rather than one.
+template <class R>
+constexpr int adj_waves()
+{
+    return sizeof(R) == 8 ? 2 : 1;
+}
+
 template <class R, class L, bool PULL>
-__global__ void __launch_bounds__(256)
-    k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, int grav_on, const R *__restrict__ A,
-                const R *__restrict__ phi, const L *__restrict__ lam, L *__restrict__ lam_out,
-                L *__restrict__ fields, double *partial, int accumulate, int remap)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves<R>())))
+    k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, Coef<L> cl, int grav_on,
+                const R *__restrict__ A, const R *__restrict__ phi, const L *__restrict__ lam,
+                L *__restrict__ lam_out, L *__restrict__ fields, double *partial, int accumulate,
+                int remap)
 {
     R ap[kNumGrad];
 #pragma unroll
@@ -290,7 +318,8 @@ __global__ void __launch_bounds__(256)
         // launch): scalar-base + vector-offset loads, fewer index registers
         const Nbr32 nb(x, y, z, nx, ny, nz);
         const uint32_t own = nb.at(0, 0, 0);
-        R p[19], fa[7], ga[19], lf[7], lg[19], afa[7], aga[19], agr[3], aphi, alap, deta;
+        R p[19], fa[7], ga[19], deta;
+        L lf[7], lg[19], afa[7], aga[19], agr[3], aphi, alap;
 #pragma unroll
         for (int d = 0; d < 19; ++d) p[d] = ld32z<0>(phi, nb.at(cx(d), cy(d), cz(d)));
         if constexpr (PULL) {
@@ -312,15 +341,15 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 1; i < 7; ++i) {
             const L *li = lam + (int64_t)i * n;
-            const R mine = ld32z<0>(li, own);
-            lfd[i] = ld32z<0>(li, nb.at(cx(i), cy(i), cz(i)));
-            lf[i] = c.eta * lfd[i] + c.eta2 * mine;
-            lfd[i] -= mine;  // for the eta derivative below
+            const L mine = ld32z<0>(li, own);
+            const L down = ld32z<0>(li, nb.at(cx(i), cy(i), cz(i)));
+            lf[i] = cl.eta * down + cl.eta2 * mine;
+            lfd[i] = (R)down - (R)mine;  // for the eta derivative below
         }
 #pragma unroll
         for (int i = 0; i < 19; ++i)
             lg[i] = ld32z<0>(lam + (int64_t)(7 + i) * n, nb.at(cx(i), cy(i), cz(i)));
-        adj_cell(c, p, fa, ga, lf, lfd, lg, afa, aga, aphi, alap, agr, deta, ap, grav_on);
+        adj_cell<R, L>(c, cl, p, fa, ga, lf, lfd, lg, afa, aga, aphi, alap, agr, deta, ap, grav_on);
         ap[kGEta] += deta;
 #pragma unroll
         for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, own, (L)(afa[i] + aphi));
@@ -555,14 +584,15 @@ struct Adj {
         const int nb = adj_blocks(g.n);
         double *partial = partials(g, scratch, grads ? 0 : 1);
         const Coef<R> c = make_coef<R>(p, grav_on);
+        const Coef<L> cl = make_coef<L>(p, grav_on);
         const int remap = 1;
         if (pulled)
             hipLaunchKernelGGL((k_adj_pass1<R, L, true>), dim3(nb), dim3(256), 0, s, g.nx, g.ny,
-                               g.nz, g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc,
+                               g.nz, g.n, c, cl, grav_on, A, phi, lam, lam_out, fields, partial, acc,
                                remap);
         else
             hipLaunchKernelGGL((k_adj_pass1<R, L, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny,
-                               g.nz, g.n, c, grav_on, A, phi, lam, lam_out, fields, partial, acc,
+                               g.nz, g.n, c, cl, grav_on, A, phi, lam, lam_out, fields, partial, acc,
                                remap);
         {
             constexpr int TY = 4;
