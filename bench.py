@@ -54,6 +54,10 @@ def parse():
                          "7-kernel step through the drop-in entry points at 256^3 FP32, "
                          "share = one GPU's z-slab of the 1024^3 box (1024x1024x128) at "
                          "FP32 and bf16 (empty disables)")
+    ap.add_argument("--placement-study", type=int, default=0, metavar="K",
+                    help="instead of the benchmark: K FP32 contexts of the box held at once "
+                         "(K lattice allocations); per allocation the linear and z-march copy "
+                         "rates between its two lattices and the fused step's time; one JSON line")
     return ap.parse_args()
 
 
@@ -262,6 +266,36 @@ def traffic_from_profile(dims):
         if len(f) == 2 and not line.startswith("#"):
             out[f[0]] = float(f[1])
     return out
+
+
+def placement_study(k: int, w, params, steps: int):
+    """The step's dependence on the allocation behind its lattices: k FP32
+    contexts held at once (k lattice pairs on different physical pages), and
+    for each the read + write rates of a linear and of a z-march copy between
+    its two lattices (zsc_copy_ceiling) and the fused step's device time on
+    the same pair."""
+    from lbm_bench_gpu_amd import zsc
+    ctxs, out = [], []
+    cells = w.cells
+    try:
+        for i in range(k):
+            print(f"# allocation {i}", file=sys.stderr, flush=True)
+            ctx = zsc.Context(*w.dims, precision="f32")
+            ctxs.append(ctx)
+            ctx.set_params(params)
+            c = ctx.copy_ceiling()
+            ctx.state_init_bubbles(w.bubbles)
+            ctx.step(2)
+            ms = ctx.step_timed(steps) / steps
+            fused = zsc.bytes_per_cell("f32") * cells / (ms * 1e-3) / 1e9
+            out.append({"copy_linear_GBs": c["linear"]["GB/s"], "copy_march_GBs": c["march"]["GB/s"],
+                        "fused_ms": round(ms, 4), "fused_GBs": round(fused, 1),
+                        "frac": round(fused / HBM_PEAK_GBS, 4),
+                        "fused_over_march": round(fused / c["march"]["GB/s"], 4)})
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+    return {"workload": w.name, "precision": "f32", "steps_timed": steps, "allocations": out}
 
 
 def cpu_model() -> str:
@@ -501,6 +535,9 @@ def main():
                      zsc.generate_bubbles(2, max(1, a.size ** 3 // 1048576), 8, 16, 4.0, 4.0, dims),
                      a.steps, ("f32",))
     params = w.params()
+    if a.placement_study > 0:
+        print(json.dumps(placement_study(a.placement_study, w, params, min(a.steps, 20))))
+        return
     print(f"# workload {w.name}: fp32 {a.steps} steps", file=sys.stderr, flush=True)
     main_r = time_precision("f32", w, params, a.steps, a.warmup, a.ty, a.zchunk, a.flags,
                             a.placement_trials)
