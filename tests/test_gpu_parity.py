@@ -187,8 +187,11 @@ def test_dropin_f32_close_to_oracle(need_gpu):
         assert rel(fa, st.f) < 1e-5
 
 
-@pytest.mark.parametrize("prec,tol_phi", [("f32", 2e-5), ("bf16", 2e-2), ("fp16", 2e-3)])
+@pytest.mark.parametrize("prec,tol_phi", [("f32", 2e-5), ("bf16", 1e-4), ("fp16", 2e-5)])
 def test_reduced_precision_vs_fp64(need_gpu, prec, tol_phi):
+    """32^3, one bubble of radius 8, 100 steps (the one-cell-per-lane kernel:
+    rows of 32 cells) against the FP64 oracle.  Measured max|dphi|: FP32
+    7.6e-6, bf16 3.1e-5, fp16 5.0e-6; the bounds leave 2.6x, 3.2x and 4x."""
     dims = (32, 32, 32)
     nx, ny, nz = dims
     b = np.array([[16.0, 16.0, 16.0, 8.0]])
