@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=512, help="cube edge (default: 512)")
-    ap.add_argument("--variants", default="bf16,fp16",
+    ap.add_argument("--variants", default="bf16,fp16,f32d",
                     help="extra storage precisions timed beside fp32")
     ap.add_argument("--ty", type=int, default=0, help="fused tile rows (0: library default)")
     ap.add_argument("--zchunk", type=int, default=-1, help="fused z chunk (-1: library default)")
@@ -66,6 +66,7 @@ STORAGE = {
     "f32": "f, g fp32 (208 B/cell)",
     "bf16": "g_i - w_i rho bf16, f_1..6 fp16, f_0 fp32 (mixed 16-bit storage, 108 B/cell)",
     "fp16": "g_i - w_i rho fp16, f_1..6 fp16, f_0 fp32 (108 B/cell)",
+    "f32d": "f fp32, g_i - w_i rho fp32 (208 B/cell)",
 }
 
 

@@ -25,7 +25,8 @@
  *
  * Beyond the drop-in kernels the library owns a fast path: a context-held
  * lattice pair advanced by zsc_step(), one fused HIP kernel per time step,
- * with FP64, FP32, or FP32-compute/16-bit-storage precisions.
+ * with FP64, FP32 (g stored as it is or as deviations from its rest value),
+ * or FP32-compute/16-bit-storage precisions.
  */
 #ifndef ZSC_H
 #define ZSC_H
@@ -52,7 +53,9 @@ typedef enum zsc_precision {
     ZSC_F64 = 0,  /* FP64 storage and compute (parity mode) */
     ZSC_F32 = 1,  /* FP32 storage and compute */
     ZSC_BF16 = 2, /* FP32 compute, populations stored as bf16 deviations */
-    ZSC_FP16 = 3  /* FP32 compute, populations stored as fp16 deviations */
+    ZSC_FP16 = 3, /* FP32 compute, populations stored as fp16 deviations */
+    ZSC_F32D = 4  /* FP32 storage and compute, g stored as FP32 deviations from
+                     w_i * rho_mean (same bytes as ZSC_F32; for large rho_mean) */
 } zsc_precision;
 
 /* ---- inputs: Input/properties.in and Input/discrete.in ----------------- */

@@ -20,7 +20,7 @@ ZSC_ERR_STATE = -5
 ZSC_ERR_DIVERGED = -6
 ZSC_ERR_PRECISION = -7
 
-PRECISIONS = {"f64": 0, "f32": 1, "bf16": 2, "fp16": 3}
+PRECISIONS = {"f64": 0, "f32": 1, "bf16": 2, "fp16": 3, "f32d": 4}
 
 _D, _I, _VP = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)

@@ -48,6 +48,7 @@ static int dispatch(int prec, F &&f)
     case ZSC_F32: return f(Tag<float>{});
     case ZSC_BF16: return f(Tag<bf16s>{});
     case ZSC_FP16: return f(Tag<f16s>{});
+    case ZSC_F32D: return f(Tag<f32d>{});
     default: return fail(ZSC_ERR_ARG, "unknown precision %d", prec);
     }
 }
@@ -150,7 +151,7 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     if (nx <= 0 || ny <= 0 || nz <= 0) return fail(ZSC_ERR_ARG, "zsc_create: sizes must be positive");
     const int64_t n = (int64_t)nx * ny * nz;
     if (n >= ((int64_t)1 << 31)) return fail(ZSC_ERR_ARG, "zsc_create: more than 2^31-1 cells");
-    if (precision < ZSC_F64 || precision > ZSC_FP16)
+    if (precision < ZSC_F64 || precision > ZSC_F32D)
         return fail(ZSC_ERR_ARG, "zsc_create: unknown precision %d", precision);
     zsc_ctx *c = new zsc_ctx;
     c->geo = {nx, ny, nz, n};
@@ -171,7 +172,7 @@ extern "C" int zsc_create(int nx, int ny, int nz, int precision, void *stream, z
     // the last of 8, run 4-5% slower)
     const int nchunk = (nz + 23) / 24;
     const int zwide = (nz + nchunk - 1) / nchunk;
-    if (precision == ZSC_F32)
+    if (precision == ZSC_F32 || precision == ZSC_F32D)
         c->cfg = {8, 64, kFusedNtStore | kFusedPrefetch};
     else if (precision == ZSC_BF16)
         c->cfg = wide ? FusedCfg{4, zwide, stash | kFusedWide} : FusedCfg{8, 32, stash};
@@ -978,7 +979,8 @@ extern "C" int zsc_copy_ceiling(zsc_ctx *c, double ms[2])
     const bool wide = c->prec == ZSC_BF16 || c->prec == ZSC_FP16;
     auto march = [&]() {
         if (c->prec == ZSC_F64) return launch_copy_march<double, double, 1>(c->geo, c->stream, c->geo.n * 8, a, b);
-        if (c->prec == ZSC_F32) return launch_copy_march<float, float, 1>(c->geo, c->stream, c->geo.n * 4, a, b);
+        if (c->prec == ZSC_F32 || c->prec == ZSC_F32D)
+            return launch_copy_march<float, float, 1>(c->geo, c->stream, c->geo.n * 4, a, b);
         return launch_copy_march<uint64_t, uint32_t, 2>(c->geo, c->stream, c->geo.n * 4, a, b);
     };
     if (chunks > 0x7fffffff) return fail(ZSC_ERR_ARG, "zsc_copy_ceiling: lattice too large");

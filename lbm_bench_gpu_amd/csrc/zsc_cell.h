@@ -330,6 +330,22 @@ __host__ __device__ inline void init_g_cell(const Coef<R> &c, const R *p, R rhon
     for (int i = 7; i < 19; ++i) g[i] = ag2;
 }
 
+// init_g_cell as deviations g_i - w_i rho_mean, from drho = rho - rho_mean:
+// the "rho" of each equilibrium becomes drho (deviation storage).
+template <class R>
+__host__ __device__ inline void init_g_cell_dev(const Coef<R> &c, const R *p, R drho, R *g)
+{
+    const R phin = p[0];
+    const R mu = chem_pot(c, phin, lap19(p));
+    const R ag1 = W<R>::w1 * (R(3) * phin * mu + drho);
+    const R ag2 = W<R>::w2 * (R(3) * phin * mu + drho);
+    g[0] = W<R>::w0 * (drho - R(6) * phin * mu);
+#pragma unroll
+    for (int i = 1; i < 7; ++i) g[i] = ag1;
+#pragma unroll
+    for (int i = 7; i < 19; ++i) g[i] = ag2;
+}
+
 // Full cell update from post-stream populations and the phi stencil:
 // update_rho / update_velocity, then collision_f and collision_g.
 template <class R>

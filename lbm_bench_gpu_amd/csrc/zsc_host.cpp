@@ -261,7 +261,8 @@ extern "C" double zsc_bytes_per_cell(int precision)
     // f_0 in FP32.
     switch (precision) {
     case ZSC_F64: return 2.0 * 26 * 8;
-    case ZSC_F32: return 2.0 * 26 * 4;
+    case ZSC_F32:
+    case ZSC_F32D: return 2.0 * 26 * 4;
     case ZSC_BF16:
     case ZSC_FP16: return 2.0 * (4 + 25 * 2);
     default: return 0.0;

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "zsc_cell.h"
 #include "zsc_launch.h"
@@ -141,9 +142,9 @@ template <int FL, class S>
 __device__ inline S ld_raw(const S *p)
 {
     if constexpr ((FL & kNtLoad) != 0) {
-        if constexpr (sizeof(S) == 2) {
+        if constexpr (std::is_class<S>::value) {  // bf16s, f16s, f32d: one scalar member
             S v;
-            v.v = __builtin_nontemporal_load((const uint16_t *)p);
+            v.v = __builtin_nontemporal_load(&p->v);
             return v;
         } else {
             return __builtin_nontemporal_load(p);
@@ -157,8 +158,8 @@ template <int FL, class S>
 __device__ inline void st_raw(S *p, S v)
 {
     if constexpr ((FL & kNtStore) != 0) {
-        if constexpr (sizeof(S) == 2)
-            __builtin_nontemporal_store(v.v, (uint16_t *)p);
+        if constexpr (std::is_class<S>::value)
+            __builtin_nontemporal_store(v.v, &p->v);
         else
             __builtin_nontemporal_store(v, p);
     } else {
@@ -204,6 +205,29 @@ __device__ inline void store_local(const Lattice<S> &L, int64_t n, int64_t idx, 
     for (int i = 1; i < 7; ++i) stg<FL>(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
 #pragma unroll
     for (int i = 0; i < 19; ++i) stg<FL>(L.g, (int64_t)i * n + idx, g[i], g_rest(c, i));
+}
+
+// load_local / store_local with g kept as stored deviations from w_i
+// rho_mean (deviation codecs), never added to and taken off the rest value.
+template <class S, class R>
+__device__ inline void load_local_dev(const Lattice<S> &L, int64_t n, int64_t idx, R *f, R *dg)
+{
+    f[0] = ldg(L.f0, idx, R(0));
+#pragma unroll
+    for (int i = 1; i < 7; ++i) f[i] = ldg(L.f, (int64_t)(i - 1) * n + idx, R(0));
+#pragma unroll
+    for (int i = 0; i < 19; ++i) dg[i] = Codec<typename Traits<S>::SG>::dec0(L.g[(int64_t)i * n + idx]);
+}
+
+template <class S, class R>
+__device__ inline void store_local_dev(const Lattice<S> &L, int64_t n, int64_t idx, const R *f,
+                                       const R *dg)
+{
+    stg(L.f0, idx, f[0], R(0));
+#pragma unroll
+    for (int i = 1; i < 7; ++i) stg(L.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
+#pragma unroll
+    for (int i = 0; i < 19; ++i) L.g[(int64_t)i * n + idx] = Codec<typename Traits<S>::SG>::enc0(dg[i]);
 }
 
 // Post-stream f of one cell, pulled from a B lattice (src/stream_f.cu:52-59
@@ -420,8 +444,13 @@ __global__ void __launch_bounds__(256)
         R p[19], f[7], g[19];
         gather19(phi, nb, p);
         init_f_cell(c, p, f);
-        init_g_cell(c, p, rho ? rho[idx] : c.rho_mean, g);
-        store_local(A, n, idx, c, f, g);
+        if constexpr (Codec<typename Traits<S>::SG>::deviation) {
+            init_g_cell_dev(c, p, rho ? rho[idx] - c.rho_mean : R(0), g);
+            store_local_dev(A, n, idx, f, g);
+        } else {
+            init_g_cell(c, p, rho ? rho[idx] : c.rho_mean, g);
+            store_local(A, n, idx, c, f, g);
+        }
     }
 }
 
@@ -454,9 +483,15 @@ __global__ void __launch_bounds__(256)
         Nbr nb(x, y, z, nx, ny, nz);
         R p[19], fa[7], ga[19], fb[7], gb[19];
         gather19(phi, nb, p);
-        load_local(A, n, idx, c, fa, ga);
-        collide_cell(c, p, fa, ga, fb, gb);
-        store_local(B, n, idx, c, fb, gb);
+        if constexpr (Codec<typename Traits<S>::SG>::deviation) {
+            load_local_dev(A, n, idx, fa, ga);
+            collide_cell_dev(c, p, fa, ga, fb, gb);
+            store_local_dev(B, n, idx, fb, gb);
+        } else {
+            load_local(A, n, idx, c, fa, ga);
+            collide_cell(c, p, fa, ga, fb, gb);
+            store_local(B, n, idx, c, fb, gb);
+        }
     }
 }
 
@@ -473,8 +508,18 @@ __global__ void __launch_bounds__(256)
         Nbr nb(x, y, z, nx, ny, nz);
         R f[7], g[19];
         pull_f(B, n, nb, c, f);
-        pull_g(B, n, nb, c, g);
-        store_local(A, n, idx, c, f, g);
+        if constexpr (Codec<typename Traits<S>::SG>::deviation) {
+            // g moves as stored
+#pragma unroll
+            for (int i = 0; i < 19; ++i)
+                A.g[(int64_t)i * n + idx] = B.g[(int64_t)i * n + nb.at(-cx(i), -cy(i), -cz(i))];
+            stg(A.f0, idx, f[0], R(0));
+#pragma unroll
+            for (int i = 1; i < 7; ++i) stg(A.f, (int64_t)(i - 1) * n + idx, f[i], R(0));
+        } else {
+            pull_g(B, n, nb, c, g);
+            store_local(A, n, idx, c, f, g);
+        }
     }
 }
 
@@ -488,9 +533,16 @@ __global__ void __launch_bounds__(256)
     using R = typename Traits<S>::R;
     ZSC_GRID_STRIDE(idx, n)
     {
-        R f[7], g[19];
-        load_local(A, n, idx, c, f, g);
-        const R r = sum_g(g);
+        R f[7], g[19], r;
+        if constexpr (Codec<typename Traits<S>::SG>::deviation) {
+            // rho = rho_mean + sum of deviations; the rest state carries no
+            // momentum (sum_i c_i w_i = 0), so u comes from the deviations
+            load_local_dev(A, n, idx, f, g);
+            r = c.rho_mean + sum_g(g);
+        } else {
+            load_local(A, n, idx, c, f, g);
+            r = sum_g(g);
+        }
         R vx, vy, vz;
         velocity(g, r, vx, vy, vz);
         if (phi) phi[idx] = sum_f(f);
@@ -508,7 +560,12 @@ __global__ void __launch_bounds__(256)
 {
     ZSC_GRID_STRIDE(idx, n)
     {
-        dst[idx] = is_f0 ? (double)ldg(f0, idx, R(0)) : (double)ldg(p, idx, rest);
+        if (is_f0)
+            dst[idx] = (double)ldg(f0, idx, R(0));
+        else if constexpr (Codec<T>::deviation)
+            dst[idx] = (double)Codec<T>::dec0(p[idx]) + (double)rest;  // exact in FP64
+        else
+            dst[idx] = (double)ldg(p, idx, rest);
     }
 }
 
@@ -521,6 +578,8 @@ __global__ void __launch_bounds__(256)
     {
         if (is_f0)
             stg(f0, idx, (R)src[idx], R(0));
+        else if constexpr (Codec<T>::deviation)
+            p[idx] = Codec<T>::enc0((R)(src[idx] - (double)rest));  // deviation rounded once
         else
             stg(p, idx, (R)src[idx], rest);
     }
@@ -890,6 +949,10 @@ struct Pair {
 };
 template <>
 struct Pair<float> {
+    using type = f32x2;
+};
+template <>
+struct Pair<f32d> {
     using type = f32x2;
 };
 

@@ -1,7 +1,7 @@
 // zsc_main.cpp -- zsc_run, the host driver of libzsc: the time loop of the
 // reference's main() (src/main.cu:83-667) over the C-ABI, on one GPU.
 //
-//   zsc_run [-d DIR] [-p f64|f32|bf16|fp16] [-v ascii|binary|none]
+//   zsc_run [-d DIR] [-p f64|f32|f32d|bf16|fp16] [-v ascii|binary|none]
 //           [-r RESTART_IN] [-s RESTART_OUT] [-t TRIALS]
 //
 // Reads DIR/properties.in and DIR/discrete.in (default: the current
@@ -15,7 +15,8 @@
 // file (no relaxation); -s writes one at the end.  -t keeps the fastest of
 // TRIALS lattice allocations (zsc_place_lattices; default 4, 0 for one
 // allocation): the step's time depends on the physical pages behind the
-// lattices.  Differences from the
+// lattices.  -p f32d stores g as FP32 deviations from w_i * rho_mean (the
+// precision to use for large density ratios).  Differences from the
 // reference: the statistics are reduced on the GPU in FP64 from the current
 // post-stream state (the reference uses moments one step old).
 #include <hip/hip_runtime.h>
@@ -142,13 +143,14 @@ int main(int argc, char **argv)
         } else if (!strcmp(argv[i], "-p") && i + 1 < argc) {
             const char *p = argv[++i];
             prec = !strcmp(p, "f64") ? ZSC_F64 : !strcmp(p, "f32") ? ZSC_F32
-                   : !strcmp(p, "bf16") ? ZSC_BF16 : !strcmp(p, "fp16") ? ZSC_FP16 : -1;
+                   : !strcmp(p, "f32d") ? ZSC_F32D : !strcmp(p, "bf16") ? ZSC_BF16
+                   : !strcmp(p, "fp16") ? ZSC_FP16 : -1;
             if (prec < 0) {
                 fprintf(stderr, "zsc_run: unknown precision %s\n", p);
                 return 2;
             }
         } else {
-            fprintf(stderr, "usage: zsc_run [-d DIR] [-p f64|f32|bf16|fp16] [-v ascii|binary|none] "
+            fprintf(stderr, "usage: zsc_run [-d DIR] [-p f64|f32|f32d|bf16|fp16] [-v ascii|binary|none] "
                             "[-r RESTART_IN] [-s RESTART_OUT] [-t TRIALS]\n");
             return 2;
         }
@@ -347,7 +349,8 @@ int main(int argc, char **argv)
         fprintf(fp, "Fused tile   = %d x %d cells, %d-plane z chunks, flags %d\n", tile_x, ty,
                 zc > 0 ? zc : in.zmax, fl);
         fprintf(fp, "Precision    = %s\n\n",
-                prec == ZSC_F64 ? "f64" : prec == ZSC_F32 ? "f32" : prec == ZSC_BF16 ? "bf16" : "fp16");
+                prec == ZSC_F64 ? "f64" : prec == ZSC_F32 ? "f32" : prec == ZSC_F32D ? "f32d"
+                : prec == ZSC_BF16 ? "bf16" : "fp16");
         // device memory the context holds (two lattices, five moment arrays
         // at the compute precision, an FP64 staging array) and the driver's
         // host buffers (bubbles; VTK output streams through 2^22-cell slabs)

@@ -4,7 +4,11 @@
 // lattices store the deviation from a per-population rest value (g_i:
 // w_i * rho_mean; f_1..6: 0) rounded to nearest-even, and decode to FP32 for
 // compute.  f_0 ~ phi is O(1) with no useful rest value, so the 16-bit
-// lattices keep f_0 in FP32 (its own array, type S0 below).
+// lattices keep f_0 in FP32 (its own array, type S0 below).  The FP32
+// deviation lattice (f32d) keeps f as it is and g as the FP32 deviation
+// from w_i * rho_mean: where rho_mean is large against the density
+// variations (rho_mean = 500.5 for a density ratio of 1000), a stored g_i
+// then keeps the bits of the variation instead of those of the rest value.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,6 +21,9 @@ struct bf16s {
 };
 struct f16s {
     uint16_t v;
+};
+struct f32d {
+    float v;
 };
 
 template <class S>
@@ -98,6 +105,16 @@ struct Codec<f16s> {
     }
 };
 
+template <>
+struct Codec<f32d> {
+    static constexpr bool deviation = true;
+    __host__ __device__ static float dec(f32d s, float rest) { return s.v + rest; }
+    __host__ __device__ static f32d enc(float v, float rest) { return {v - rest}; }
+    __host__ __device__ static float dec0(f32d s) { return s.v; }
+    __host__ __device__ static float dec0(float s) { return s; }  // an element of a float pair
+    __host__ __device__ static f32d enc0(float v) { return {v}; }
+};
+
 // Compute type R and the storage types of f_0 (S0), f_1..6 (SF) and g (SG)
 // for a precision tag S.  bf16 mode keeps the f populations, which carry
 // phi, in fp16 (four times finer than bf16 at the same 2 bytes) and g in
@@ -107,6 +124,7 @@ template <> struct Traits<double> { using R = double; using S0 = double; using S
 template <> struct Traits<float> { using R = float; using S0 = float; using SF = float; using SG = float; };
 template <> struct Traits<bf16s> { using R = float; using S0 = float; using SF = f16s; using SG = bf16s; };
 template <> struct Traits<f16s> { using R = float; using S0 = float; using SF = f16s; using SG = f16s; };
+template <> struct Traits<f32d> { using R = float; using S0 = float; using SF = float; using SG = f32d; };
 
 // One lattice (A or B) in the context's layout: f_0 array, f_1..6 with
 // stride n, g_0..18 with stride n.  Pointers are device addresses.

@@ -59,7 +59,8 @@ def test_library_errors_without_a_context():
     assert L.zsc_step(None, 1) == _lib.ZSC_ERR_ARG
     assert b"null context" in L.zsc_last_error()
     assert L.zsc_create(0, 4, 4, 0, None, ctypes.byref(ctypes.c_void_p())) == _lib.ZSC_ERR_ARG
-    assert L.zsc_create(4, 4, 4, 9, None, ctypes.byref(ctypes.c_void_p())) == _lib.ZSC_ERR_ARG
+    for bad in (-1, 5, 9):  # ZSC_F64 .. ZSC_F32D are 0 .. 4
+        assert L.zsc_create(4, 4, 4, bad, None, ctypes.byref(ctypes.c_void_p())) == _lib.ZSC_ERR_ARG
     assert L.zsc_bytes_per_cell(7) == 0.0
 
 

@@ -1,12 +1,14 @@
-"""Known-answer tests of the 16-bit population codecs (bf16 and fp16
-deviation storage) through the C-ABI: zsc_state_set encodes FP64 host values
+"""Known-answer tests of the deviation codecs (bf16, fp16 and FP32 storage
+of g as deviations) through the C-ABI: zsc_state_set encodes FP64 host values
 into the context's lattice, zsc_state_get decodes them back.
 
 The expected values are formed here, independently of the library:
-  g_i (rest r_i = fl32(w_i) * fl32(rho_mean) in FP32):
-      stored  = RNE16(fl32(fl32(v) - r_i)),  read = fl32(stored + r_i)
-  f_1..6 (rest 0):  stored = RNE16(fl32(v)),  read = stored
-  f_0 (FP32 in both modes):  read = fl32(v)
+  g_i (rest r_i = fl32(w_i) * fl32(rho_mean) in FP32; v - r_i and
+  stored + r_i in FP64, where both are exact):
+      bf16 / fp16: stored = RNE16(fl32(v - r_i)),  read = stored + r_i
+      f32d:        stored = fl32(v - r_i),         read = stored + r_i
+  f_1..6 (rest 0):  stored = RNE16(fl32(v)) (16-bit modes), fl32(v) (f32d)
+  f_0 (FP32 in every mode):  read = fl32(v)
 RNE16 is round-to-nearest-even to bf16 (torch's float32 -> bfloat16
 conversion) or to fp16 (NumPy's float32 -> float16 conversion), and the
 tie and subnormal vectors below also carry their rounded value in closed
@@ -66,18 +68,34 @@ def fp16_subnormal_cases():
     return np.array(d), np.array(want)
 
 
+def rest_of(i, rho_mean):
+    return np.float64(F32(W[i]) * F32(rho_mean))
+
+
 def expected_round_trip(prec, f, g, rho_mean):
     fe = np.empty_like(f)
     ge = np.empty_like(g)
     fe[0] = f[0].astype(F32)
-    fe[1:] = fp16_rne(f[1:].astype(F32)).astype(np.float64)
-    enc = bf16_rne if prec == "bf16" else fp16_rne
+    if prec == "f32d":
+        fe[1:] = f[1:].astype(F32)
+        enc = lambda d: d
+    else:
+        fe[1:] = fp16_rne(f[1:].astype(F32)).astype(np.float64)
+        enc = bf16_rne if prec == "bf16" else fp16_rne
     with np.errstate(invalid="ignore", over="ignore"):
         for i in range(19):
-            rest = F32(W[i]) * F32(rho_mean)
-            dev = (g[i].astype(F32) - rest).astype(F32)
-            ge[i] = (enc(dev) + rest).astype(F32)
+            rest = rest_of(i, rho_mean)
+            dev = (g[i] - rest).astype(F32)
+            ge[i] = enc(dev).astype(np.float64) + rest
     return fe, ge
+
+
+def compare(prec, f, g, fe, ge, fo, go):
+    bad_f = np.argwhere(~((fo == fe) | (np.isnan(fo) & np.isnan(fe))))
+    bad_g = np.argwhere(~((go == ge) | (np.isnan(go) & np.isnan(ge))))
+    print(prec, "codec mismatches f", len(bad_f), "g", len(bad_g), flush=True)
+    assert len(bad_f) == 0, [(tuple(k), f[tuple(k)], fo[tuple(k)], fe[tuple(k)]) for k in bad_f[:5]]
+    assert len(bad_g) == 0, [(tuple(k), g[tuple(k)], go[tuple(k)], ge[tuple(k)]) for k in bad_g[:5]]
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
@@ -107,7 +125,7 @@ def test_codec_known_answers(need_gpu, prec):
     # the closed forms of the tie / subnormal vectors agree with the
     # expectation built from the library-independent converters
     for i in range(19):
-        assert np.array_equal(ge[i, : dg.size], (F32(rests[i]) + wg.astype(F32)).astype(F32)), i
+        assert np.array_equal(ge[i, : dg.size], rests[i] + wg), i
     assert np.array_equal(fe[1, : df.size], wf.astype(F32).astype(np.float64))
 
     with zsc.Context(*dims, precision=prec) as ctx:
@@ -115,9 +133,36 @@ def test_codec_known_answers(need_gpu, prec):
         ctx.state_init(phi0)
         ctx.state_set(f.reshape(7, nz, ny, nx), g.reshape(19, nz, ny, nx))
         fo, go = ctx.state_get()
+    compare(prec, f, g, fe, ge, fo.reshape(7, n), go.reshape(19, n))
+
+
+def test_f32d_keeps_the_deviation_bits(need_gpu):
+    """FP32 storage of g as deviations at rho_mean = 500.5 (the reference's
+    Input/properties.in, density ratio 1000): values rest + d with |d| from
+    1e-12 to 1e-2 read back as rest + fl32(d) exactly, i.e. within 2^-24 |d|
+    of the value written, where an FP32 store of the value itself rounds at
+    ulp(w_i * 500.5) (up to 2^-18); f, NaN and +-Inf as in the other codecs."""
+    dims = (16, 8, 4)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, seed=43)
+    p = p.replace(rho_mean=500.5)
+    rng = np.random.default_rng(44)
+    f = rng.normal(0.0, 1e-3, (7, n))
+    f[0] = rng.uniform(-499.5, 499.5, n)
+    d = rng.choice([-1.0, 1.0], (19, n)) * 10.0 ** rng.uniform(-12, -2, (19, n))
+    g = np.array([rest_of(i, 500.5) + d[i] for i in range(19)])
+    g[4, :3] = [np.nan, np.inf, -np.inf]
+    fe, ge = expected_round_trip("f32d", f, g, 500.5)
+    with zsc.Context(*dims, precision="f32d") as ctx:
+        ctx.set_params(p)
+        ctx.state_init(phi0)
+        ctx.state_set(f.reshape(7, nz, ny, nx), g.reshape(19, nz, ny, nx))
+        fo, go = ctx.state_get()
     fo, go = fo.reshape(7, n), go.reshape(19, n)
-    bad_f = np.argwhere(~((fo == fe) | (np.isnan(fo) & np.isnan(fe))))
-    bad_g = np.argwhere(~((go == ge) | (np.isnan(go) & np.isnan(ge))))
-    print(prec, "codec mismatches f", len(bad_f), "g", len(bad_g), flush=True)
-    assert len(bad_f) == 0, [(tuple(k), f[tuple(k)], fo[tuple(k)], fe[tuple(k)]) for k in bad_f[:5]]
-    assert len(bad_g) == 0, [(tuple(k), g[tuple(k)], go[tuple(k)], ge[tuple(k)]) for k in bad_g[:5]]
+    compare("f32d", f, g, fe, ge, fo, go)
+    ok = np.isfinite(g)
+    assert np.all(np.abs(go - g)[ok] <= 2.0 ** -24 * np.abs(d)[ok])
+    raw = np.array([g[i].astype(F32).astype(np.float64) for i in range(19)])
+    print("f32d max |read - written| / |d|", float(np.max((np.abs(go - g) / np.abs(d))[ok])),
+          "an FP32 store of the value:", float(np.max((np.abs(raw - g) / np.abs(d))[ok])), flush=True)

@@ -42,7 +42,7 @@ def test_dropin_and_adjoint_need_f32_or_f64(need_gpu):
     p, _ = small_case(dims, seed=32)
     f = [torch.zeros(2 * n, dtype=torch.float32, device="cuda") for _ in range(7)]
     phi = torch.zeros(n, dtype=torch.float32, device="cuda")
-    for prec in ("bf16", "fp16"):
+    for prec in ("bf16", "fp16", "f32d"):
         with zsc.Context(*dims, precision=prec) as ctx:
             ctx.set_params(p)
             assert _code(ctx.init_f, phi, f) == ZSC_ERR_PRECISION
@@ -60,7 +60,7 @@ def test_bad_launch_shape_is_refused(need_gpu):
         assert _code(ctx.set_fused_config, 4, 16, 200 << 8) == ZSC_ERR_ARG  # LDS pad > 128 KiB
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32d", "bf16"])
 def test_diverged_state_is_reported(need_gpu, prec):
     dims = (8, 6, 4)
     p, phi0 = small_case(dims, seed=33)
@@ -77,7 +77,8 @@ def test_diverged_state_is_reported(need_gpu, prec):
 
 
 @pytest.mark.parametrize("prec,dims", [("f32", (64, 16, 8)), ("bf16", (128, 8, 8)),
-                                       ("fp16", (63, 5, 3)), ("f64", (16, 8, 4))])
+                                       ("fp16", (63, 5, 3)), ("f64", (16, 8, 4)),
+                                       ("f32d", (33, 5, 3))])
 def test_copy_ceiling(need_gpu, prec, dims):
     """zsc_copy_ceiling times both copies on the context's lattices (the
     march copy only where the x extent suits its lane words) and refuses to
@@ -87,7 +88,7 @@ def test_copy_ceiling(need_gpu, prec, dims):
         ctx.set_params(p)
         c = ctx.copy_ceiling()
         assert c["linear"]["ms"] > 0 and c["bytes_per_copy"] > 0
-        assert (c["march"]["ms"] > 0) == (prec in ("f32", "f64") or dims[0] % 2 == 0)
+        assert (c["march"]["ms"] > 0) == (prec in ("f32", "f32d", "f64") or dims[0] % 2 == 0)
         ctx.state_init(phi0)
         assert _code(ctx.copy_ceiling) == ZSC_ERR_STATE
         ctx.step(1)

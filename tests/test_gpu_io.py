@@ -115,7 +115,7 @@ def test_vtk_binary_holds_the_fields(need_gpu, tmp_path, prec):
         assert np.array_equal(got_u[:, a], f32(u))
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32d", "bf16", "fp16"])
 def test_restart_continues_bit_for_bit(need_gpu, tmp_path, prec):
     dims = (70, 12, 10)
     p, phi0 = small_case(dims, seed=25)
@@ -141,7 +141,7 @@ def test_restart_continues_bit_for_bit(need_gpu, tmp_path, prec):
         assert np.array_equal(f, f_ref) and np.array_equal(g, g_ref)
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["f32", "f32d", "bf16", "fp16"])
 def test_reading_the_state_does_not_change_the_run(need_gpu, prec):
     """Moments, statistics, pressure and populations are read from a streamed
     copy; the held post-collision state goes on unchanged."""
@@ -355,3 +355,49 @@ def test_driver_reduced_precision_with_placement_trials(need_gpu, tmp_path, prec
     err = float(np.abs(got - st.moments()[0].ravel()).max())
     print(prec, "zsc_run max|dphi| vs oracle", err, flush=True)
     assert err <= tol
+
+
+def test_driver_on_the_reference_input_files(need_gpu, tmp_path):
+    """zsc_run on Input/properties.in and Input/discrete.in exactly as the
+    reference ships them (128^3, one R = 10 bubble, rhoH/rhoL = 1000; 1000
+    relaxation and 999 evolution steps) at FP64, at FP32 (the reference's own
+    precision) and at FP32 with g stored as deviations (f32d): the stats
+    lines the reference writes, the runlog sections, and the FP32 and f32d
+    bubble statistics against FP64's: the same volume and radius to the
+    printed digits;
+    the bubble's mean rise velocity and the maximum velocity within 7e-2
+    (FP32: measured 2.3e-2, 1.6e-2) and 5e-5 (f32d: 1.7e-5, 1.4e-5) of FP64's
+    largest."""
+    import shutil
+    from tests.test_gpu_parity import REF_IN
+    out = {}
+    for prec in ("f64", "f32", "f32d"):
+        d = tmp_path / prec
+        d.mkdir()
+        for f in ("properties.in", "discrete.in"):
+            shutil.copy(os.path.join(REF_IN, f), d / f)
+        _driver(["-d", str(d), "-p", prec, "-v", "none"], str(d))
+        relax, evo = np.loadtxt(d / "relax_stats.out"), np.loadtxt(d / "stats.out")
+        assert relax[:, 0].tolist() == list(range(0, 1001, 100))
+        assert evo[:, 0].tolist() == list(range(0, 1000, 100))
+        log = open(d / "runlog.out").read()
+        for sec in ("INPUT PARAMETERS", "RELAXATION RESULTS", "OUTPUT RESULTS",
+                    "TIMING INFORMATION", "*** Simulation completed at"):
+            assert sec in log, sec
+        out[prec] = (relax, evo, log)
+        print(prec, "relax last", relax[-1].tolist(), "evo last", evo[-1].tolist(), flush=True)
+        relax_block = log.split("RELAXATION RESULTS")[1].split("\n\n")[0]
+        for key in ("(Pin - Pout)", "Laplace Error", "Effective Radius", "Parasitic Velocity"):
+            print(prec, key, _log_value(relax_block, key), flush=True)
+    e64 = out["f64"][1]
+    for prec in ("f32", "f32d"):
+        e = out[prec][1]
+        # columns: step, bubble-mean ux uy uz, Umax, Vef, Ref
+        d_vef = float(np.abs(e[:, 5] - e64[:, 5]).max())
+        d_ref = float(np.abs(e[:, 6] - e64[:, 6]).max())
+        d_uz = float(np.abs(e[1:, 3] - e64[1:, 3]).max() / np.abs(e64[1:, 3]).max())
+        d_um = float(np.abs(e[1:, 4] - e64[1:, 4]).max() / np.abs(e64[1:, 4]).max())
+        print(prec, "vs f64: |dVef|", d_vef, "|dRef|", d_ref, "rel duz", d_uz, "rel dUmax", d_um,
+              flush=True)
+        bound = 7e-2 if prec == "f32" else 5e-5
+        assert d_vef <= 1e-6 and d_ref <= 1e-5 and d_uz <= bound and d_um <= bound

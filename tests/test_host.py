@@ -177,7 +177,7 @@ def test_host_init_matches_oracle(i):
 
 def test_bytes_per_cell():
     assert zsc.bytes_per_cell("f64") == 416
-    assert zsc.bytes_per_cell("f32") == 208
+    assert zsc.bytes_per_cell("f32") == 208 and zsc.bytes_per_cell("f32d") == 208
     assert zsc.bytes_per_cell("bf16") == 108 and zsc.bytes_per_cell("fp16") == 108
 
 
