@@ -3,7 +3,8 @@
 
 Workload: a 512^3 periodic box with 128 bubbles (R ~ U[8,16], seed 2) and the
 ZSC physics of lbm_bench_gpu_amd.configs -- FP32 populations for the headline
-value, plus the same box with bf16 and fp16 deviation storage.  A step is one
+value, plus the same box with bf16 and fp16 deviation storage and with FP32
+storage of g as deviations (f32d).  A step is one
 reference time step (update, collide, stream), which is one launch of the
 fused kernel.  Inputs are resident in HBM before the timed region.
 
@@ -209,7 +210,11 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     modes = (("tape", {"tape": "auto"}), ("checkpoints", {"tape": False}),
              # FP32 adjoint lattices over FP64 states: the FP64 forward's
              # gradients (1e-7 of central differences) at FP32 adjoint storage
-             ("checkpoints_f32_over_f64_states", {"tape": False, "states": "f64"}))
+             ("checkpoints_f32_over_f64_states", {"tape": False, "states": "f64"}),
+             # every lattice FP32, g of the states stored as deviations from
+             # w_i rho_mean: gradients within 1e-4 of central differences at
+             # the FP32 modes' cost
+             ("checkpoints_f32_over_f32d_states", {"tape": False, "states": "f32d"}))
     for mode, kw in modes:
         run = adjoint.AdjointRunner(w.dims, p, precision="f32", checkpoint_every=20, timing=True,
                                     **kw)

@@ -308,19 +308,25 @@ int zsc_set_fused_config(zsc_ctx *ctx, int ty, int zchunk, int flags);
 int zsc_get_fused_config(zsc_ctx *ctx, int *ty, int *zchunk, int *flags);
 
 /* ---- adjoint (no reference counterpart: the reference has no derivatives) */
-/* Hand-written reverse mode of the time step for F64 and F32 contexts.  All
- * pointers are device pointers; lattices are plain pop-major (26, N) arrays
- * of the context's element type (f_0..f_6 then g_0..g_18, population q at
- * element q*N); phi arrays hold N elements; grads is a device double[16]
- * that the calls accumulate into (index: 0 alpha4, 1 kappa, 2 inv_tau_phi_one,
- * 3 inv_tau_phi, 4 inv_eta2, 5 eta, 6 inv_tau_rho_one, 7-9 k0..k2,
- * 10-12 kc0..kc2, 13 grav, 14 the loss).  The gravity switch applies as in
+/* Hand-written reverse mode of the time step for F64, F32 and F32D
+ * contexts.  All pointers are device pointers; lattices are plain pop-major
+ * (26, N) arrays of the context's element type (f_0..f_6 then g_0..g_18,
+ * population q at element q*N; float for F32D, whose g_i are stored as
+ * g_i - w_i * rho_mean); phi arrays hold N elements; grads is a device
+ * double[16] that the calls accumulate into (index: 0 alpha4, 1 kappa,
+ * 2 inv_tau_phi_one, 3 inv_tau_phi, 4 inv_eta2, 5 eta, 6 inv_tau_rho_one,
+ * 7-9 k0..k2, 10-12 kc0..kc2, 13 grav, 14 the loss).  Over F32D states the
+ * sums 6-9 leave out the rest state's part (rho_mean times the lambda sums
+ * weighted by w_i, resp. over each k's population group), which cancels in
+ * every derivative through tau_g; that cancellation is what FP32 states lose
+ * accuracy to, and F32D states keep tau_g's gradient within 1e-4.  The gravity switch applies as in
  * zsc_step.  A context's scratch buffer of zsc_adj_scratch_bytes() is passed
  * in by the caller, which owns all memory. */
 size_t zsc_adj_scratch_bytes(zsc_ctx *ctx);
 /* Storage of the adjoint lattices (lam arguments of zsc_adj_seed,
  * zsc_adj_backward, zsc_adj_init_backward): the context's own precision
- * (default), or ZSC_F32 over an F64 context -- FP32 adjoint lattices over
+ * (default; ZSC_F32 and ZSC_F32D both mean FP32 adjoint lattices over an
+ * F32D context), or ZSC_F32 over an F64 context -- FP32 adjoint lattices over
  * FP64 states, with the per-cell reverse arithmetic and the coefficient
  * sums in FP64.  That mode keeps the FP64 forward's accuracy where FP32
  * states cannot: the tau_g derivative pairs sum_i lam_i g_i against

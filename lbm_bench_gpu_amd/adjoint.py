@@ -85,14 +85,19 @@ class AdjointRunner:
         the forward run and its stored states (default: the same).  FP32
         adjoints over FP64 states (precision="f32", states="f64") keep the
         FP64 forward and FP64 per-cell reverse arithmetic at FP32 adjoint
-        storage.  tape: True keeps every step's input state and phi (no
+        storage.  FP32 adjoints over f32d states (states="f32d": FP32
+        lattices whose g are deviations from w_i rho_mean) run at FP32 cost
+        throughout and keep the density variations that FP32 states round
+        away, with the coefficient sums formed without the rest state's
+        part (it cancels in the tau_g derivative).  tape: True keeps every step's input state and phi (no
         recomputation), False a checkpoint every ``checkpoint_every`` steps,
         "auto" the tape when it takes at most ``tape_fraction`` of the device
         memory that is free (to the driver or in torch's cache) when the
         forward pass starts."""
         states = states or precision
-        if (precision, states) not in (("f32", "f32"), ("f64", "f64"), ("f32", "f64")):
-            raise ValueError("the adjoint runs at f32 or f64, or f32 over f64 states")
+        if (precision, states) not in (("f32", "f32"), ("f64", "f64"), ("f32", "f64"),
+                                       ("f32", "f32d")):
+            raise ValueError("the adjoint runs at f32 or f64, or f32 over f64 or f32d states")
         if tape not in (True, False, "auto"):
             raise ValueError("tape is True, False or 'auto'")
         if int(checkpoint_every) < 1:
@@ -105,7 +110,7 @@ class AdjointRunner:
         self.n = nx * ny * nz
         self.base = base
         self.k = int(checkpoint_every)
-        self.dtype = torch.float32 if states == "f32" else torch.float64    # states, phi
+        self.dtype = torch.float64 if states == "f64" else torch.float32    # states, phi
         self.ldtype = torch.float32 if precision == "f32" else torch.float64  # adjoint lattices
         self.device = device
         # A stream of our own (default priority): the library's kernels and

@@ -62,14 +62,24 @@ enum : int {
 // R = float) keep in FP64 what cancels: the coefficient sums pair large
 // primal values (sum_i lam_i g_i against lambda-weighted equilibria) whose
 // difference is ~1e-4 of either.
-template <class P, class R>
+//
+// DEV: ga holds deviations g_i - w_i rho_mean (FP32 states stored so, as
+// the f32d lattice keeps them).  rho = rho_mean + sum of deviations, the
+// momenta are the same sums (sum_i c_i w_i = 0), and the coefficient sums
+// of inv_tau_rho_one (sum_i g_i lam_i) and of k0..k2 (the "rho * 1" terms
+// of the equilibria) leave out the rest state's part, rho_mean sum_i w_i
+// lam_i: through tau_g (inv_tau_rho_one = 1 - 1/tau_g, k_i = w_i/tau_g) the
+// two parts cancel exactly, and without them the sums keep the bits that
+// FP32 would lose to that cancellation.
+template <class P, class R, bool DEV = false>
 __device__ inline void adj_cell(const Coef<P> &c, const Coef<R> &cr, const P *p, const P *fa,
                                 const P *ga, const R *lf, const P *lfd, const R *lg, R *afa,
                                 R *aga, R &aphi, R &alap, R *agr, P &deta, P *ap, int grav_on)
 {
     // ---- forward recompute
     const P phi = p[0];
-    const P rho = sum_g(ga);
+    const P gsum = sum_g(ga);
+    const P rho = DEV ? c.rho_mean + gsum : gsum;
     const P jx = ga[1] - ga[2] + ga[7] - ga[8] + ga[9] - ga[10] + ga[11] - ga[12] + ga[13] - ga[14];
     const P jy = ga[3] - ga[4] + ga[7] - ga[8] - ga[9] + ga[10] + ga[15] - ga[16] + ga[17] - ga[18];
     const P jz = ga[5] - ga[6] + ga[11] - ga[12] - ga[13] + ga[14] + ga[15] - ga[16] - ga[17] + ga[18];
@@ -136,14 +146,19 @@ __device__ inline void adj_cell(const Coef<P> &c, const Coef<R> &cr, const P *p,
     const P ut[3] = {P(3) * v[0], P(3) * v[1], P(3) * v[2]};
     const P UF = v[0] * F[0] + v[1] * F[1] + v[2] * F[2];
     const P Q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-    const P X = P(3) * phi * mu + rho * (P(1) - P(1.5) * Q);
+    P X;
+    if constexpr (DEV)
+        X = P(3) * phi * mu + gsum - P(1.5) * rho * Q;  // drho for rho * 1
+    else
+        X = P(3) * phi * mu + rho * (P(1) - P(1.5) * Q);
     const R Q_r = (R)Q;
     const R utr[3] = {(R)ut[0], (R)ut[1], (R)ut[2]}, Fr[3] = {(R)F[0], (R)F[1], (R)F[2]};
     R a_UF = R(0), a_Q = R(0), a_X = R(0), a_ut[3] = {0, 0, 0};
     P g_k0, g_k1 = P(0), g_k2 = P(0), g_kc0, g_kc1 = P(0), g_kc2 = P(0);
     {
         const R D = lg[0];
-        g_k0 = (rho - P(6) * phi * mu - P(1.5) * rho * Q) * (P)D;
+        // the "rho * 1" term: drho (= gsum) for deviation states
+        g_k0 = (gsum - P(6) * phi * mu - P(1.5) * rho * Q) * (P)D;
         a_rho += cr.k0 * (R(1) - R(1.5) * Q_r) * D;
         a_phi += R(-6) * cr.k0 * mu_r * D;
         a_mu += R(-6) * cr.k0 * phi_r * D;
@@ -300,7 +315,7 @@ constexpr int adj_waves()
     return sizeof(R) == 8 ? 2 : 1;
 }
 
-template <class R, class L, bool PULL>
+template <class R, class L, bool PULL, bool DEV = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves<R>())))
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, Coef<L> cl, int grav_on,
                 const R *__restrict__ A, const R *__restrict__ phi, const L *__restrict__ lam,
@@ -349,7 +364,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_wa
 #pragma unroll
         for (int i = 0; i < 19; ++i)
             lg[i] = ld32z<0>(lam + (int64_t)(7 + i) * n, nb.at(cx(i), cy(i), cz(i)));
-        adj_cell<R, L>(c, cl, p, fa, ga, lf, lfd, lg, afa, aga, aphi, alap, agr, deta, ap, grav_on);
+        adj_cell<R, L, DEV>(c, cl, p, fa, ga, lf, lfd, lg, afa, aga, aphi, alap, agr, deta, ap,
+                            grav_on);
         ap[kGEta] += deta;
 #pragma unroll
         for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, own, (L)(afa[i] + aphi));
@@ -570,13 +586,14 @@ struct Adj {
         return (double *)((char *)scratch + field_bytes(g.n)) + (size_t)which * kAdjMaxBlocks * kNumGrad;
     }
     // A: the step's input state, or (pulled) the previous post-collision
-    // lattice, streamed on the fly.
+    // lattice, streamed on the fly; dev: its g are deviations from w_i
+    // rho_mean (FP32 only).
     // grads == nullptr defers the coefficient sums into per-block partials
     // kept in the scratch buffer until flush(); defer_first: no deferred
     // sums exist yet, so the partials are written rather than added to.
     static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
                                const R *A, bool pulled, const R *phi, const L *lam, L *lam_out,
-                               void *scratch, double *grads, bool defer_first)
+                               void *scratch, double *grads, bool defer_first, bool dev = false)
     {
         if ((uint64_t)g.n * sizeof(R) > ((uint64_t)1 << 32)) return hipErrorInvalidValue;
         L *fields = (L *)scratch;
@@ -586,14 +603,16 @@ struct Adj {
         const Coef<R> c = make_coef<R>(p, grav_on);
         const Coef<L> cl = make_coef<L>(p, grav_on);
         const int remap = 1;
-        if (pulled)
-            hipLaunchKernelGGL((k_adj_pass1<R, L, true>), dim3(nb), dim3(256), 0, s, g.nx, g.ny,
-                               g.nz, g.n, c, cl, grav_on, A, phi, lam, lam_out, fields, partial, acc,
-                               remap);
+        auto pass1 = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n, c, cl,
+                               grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
+        };
+        if (!dev)
+            pulled ? pass1(k_adj_pass1<R, L, true>) : pass1(k_adj_pass1<R, L, false>);
+        else if constexpr (sizeof(R) == 4)  // deviation states exist at FP32 only
+            pulled ? pass1(k_adj_pass1<R, L, true, true>) : pass1(k_adj_pass1<R, L, false, true>);
         else
-            hipLaunchKernelGGL((k_adj_pass1<R, L, false>), dim3(nb), dim3(256), 0, s, g.nx, g.ny,
-                               g.nz, g.n, c, cl, grav_on, A, phi, lam, lam_out, fields, partial, acc,
-                               remap);
+            return hipErrorInvalidValue;
         {
             constexpr int TY = 4;
             const int nbx = (g.nx + 63) / 64, nby = (g.ny + TY - 1) / TY;

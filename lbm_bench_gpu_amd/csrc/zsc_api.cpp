@@ -1308,35 +1308,54 @@ extern "C" int zsc_adj_set_lambda_precision(zsc_ctx *c, int precision)
         c->adj_lam_f32 = false;
         return ZSC_OK;
     }
+    if (c->prec == ZSC_F32D && (precision == ZSC_F32 || precision == ZSC_F32D)) {
+        c->adj_lam_f32 = false;  // FP32 adjoint lattices over FP32 deviation states
+        return ZSC_OK;
+    }
     if (c->prec == ZSC_F64 && precision == ZSC_F32) {
         c->adj_lam_f32 = true;
         return ZSC_OK;
     }
     return fail(ZSC_ERR_PRECISION, "zsc_adj_set_lambda_precision: adjoint lattices at precision %d "
-                                   "over a context at %d (F64 or F32 as the context, or F32 over F64)",
+                                   "over a context at %d (F64 or F32 as the context, F32 over F64 "
+                                   "or over F32D)",
                 precision, c->prec);
 }
 
+// f(tag of the context's lattice type): double, float, or f32d (FP32
+// lattices whose g are deviations from w_i rho_mean, same (26, N) layout).
 template <class F>
 static int adj_call(zsc_ctx *c, const char *name, F &&f)
 {
     if (int r = need_ctx(c)) return r;
     if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
+    if (c->prec == ZSC_F32D) {
+        if (int r = set_device(c)) return r;
+        const hipError_t e = f(Tag<f32d>{});
+        return e == hipSuccess ? ZSC_OK : hip_fail(e, name);
+    }
     return dropin(c, false, name, std::forward<F>(f));
 }
 
 // adj_call for the entry points that take adjoint lattices: f(state tag,
-// adjoint-lattice tag).
+// adjoint-lattice tag).  F32D states run the FP32 adjoint, told by
+// adj_dev(c) that their g are deviations.
 template <class F>
 static int adj_lam_call(zsc_ctx *c, const char *name, F &&f)
 {
     return adj_call(c, name, [&](auto tag) -> hipError_t {
         using T = ZSC_T;
-        if constexpr (std::is_same<T, double>::value)
-            if (c->adj_lam_f32) return f(tag, Tag<float>{});
-        return f(tag, tag);
+        if constexpr (std::is_same<T, f32d>::value) {
+            return f(Tag<float>{}, Tag<float>{});
+        } else {
+            if constexpr (std::is_same<T, double>::value)
+                if (c->adj_lam_f32) return f(tag, Tag<float>{});
+            return f(tag, tag);
+        }
     });
 }
+
+static bool adj_dev(const zsc_ctx *c) { return c->prec == ZSC_F32D; }
 
 extern "C" int zsc_adj_init(zsc_ctx *c, const void *phi0, void *a0)
 {
@@ -1399,7 +1418,7 @@ extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const
         hipError_t e = Adj<T, L>::backward(c->geo, c->params, c->grav_on, c->stream,
                                            (const T *)state, pulled != 0, (const T *)phi_n,
                                            (const L *)lam_next, (L *)lam_out, adj_scr, grads,
-                                           !grads && !c->adj_deferred);
+                                           !grads && !c->adj_deferred, adj_dev(c));
         if (e == hipSuccess && !grads) c->adj_deferred = true;
         return e;
     });

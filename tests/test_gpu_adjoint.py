@@ -22,8 +22,9 @@ def _vp(t):
 
 
 def _runner_kw(precision):
-    """AdjointRunner arguments of a test precision: "f64", "f32", or
-    "f32/f64" (FP32 adjoint lattices over FP64 states)."""
+    """AdjointRunner arguments of a test precision: "f64", "f32", "f32/f64"
+    (FP32 adjoint lattices over FP64 states) or "f32/f32d" (over FP32 states
+    whose g are deviations from w_i rho_mean)."""
     lam, _, states = precision.partition("/")
     return {"precision": lam, "states": states or lam}
 
@@ -98,7 +99,7 @@ def test_fp64_gradients_match_taped_cpu(need_gpu, tape):
         assert abs(a - r) <= 1e-8 * max(abs(r), 1e-30), (a, r)
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64", "f32/f32d"])
 def test_tape_equals_recomputation(need_gpu, precision):
     """The tape holds the same states and phi_n that the checkpointed
     backward pass recomputes (the same kernels on the same inputs), so both
@@ -151,10 +152,11 @@ def _workload_fd(precision, gravity_on):
 
 SMOOTH_TOL = {"f64": {"kappa": 1e-6, "beta": 1e-6, "tau_f": 1e-6, "tau_g": 1e-6},
               "f32/f64": {"kappa": 1e-4, "beta": 1e-4, "tau_f": 1e-4, "tau_g": 1e-4},
-              "f32": {"kappa": 1e-3, "beta": 1e-3, "tau_f": 2e-2, "tau_g": 5e-2}}
+              "f32": {"kappa": 1e-3, "beta": 1e-3, "tau_f": 2e-2, "tau_g": 5e-2},
+              "f32/f32d": {"kappa": 1e-6, "beta": 1e-6, "tau_f": 1.5e-4, "tau_g": 1e-4}}
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32/f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32/f64", "f32", "f32/f32d"])
 def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     """The 128^3 workload (8 bubbles R = 12, seed 4, 400 steps, target from
     kappa x1.2, beta x0.9) with gravity switched off, so the loss is smooth
@@ -169,8 +171,11 @@ def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     non-equilibrium part FP32 states (g stored absolutely) carry to ~1e-3
     per cell: ~1.5% off.  FP32 adjoint lattices over FP64 states ("f32/f64":
     FP64 forward and per-cell reverse arithmetic, FP32 lambda storage) meet
-    1e-4 for every parameter.  The FP32 bounds are those measurements with
-    room; the gravity derivative is 0."""
+    1e-4 for every parameter.  So do FP32 adjoints over f32d states
+    ("f32/f32d": every lattice FP32, g stored as deviations from w_i
+    rho_mean, the tau_g sums formed without the rest state's part):
+    4.4e-7, 1.9e-7, 6.6e-5, 4.6e-5.  The FP32 bounds are those measurements
+    with room; the gravity derivative is 0."""
     grads, fd = _workload_fd(precision, False)
     errs = {name: abs(a - r) / abs(r) for name, a, r in list(zip(adjoint.THETA, grads, fd))[:4]}
     print(precision, "relative error vs FD", errs, flush=True)
@@ -178,7 +183,8 @@ def test_adjoint_workload_smooth_vs_central_differences(need_gpu, precision):
     assert grads[4] == 0.0
 
 
-@pytest.mark.parametrize("precision,tol", [("f64", 5e-4), ("f32/f64", 5e-4), ("f32", 3e-3)])
+@pytest.mark.parametrize("precision,tol", [("f64", 5e-4), ("f32/f64", 5e-4), ("f32", 3e-3),
+                                           ("f32/f32d", 5e-4)])
 def test_adjoint_workload_vs_central_differences(need_gpu, precision, tol):
     """The 128^3 workload as specified (gravity x1.5 in the target).  The
     phi > 0 gravity switch makes the loss piecewise smooth: central
@@ -191,7 +197,7 @@ def test_adjoint_workload_vs_central_differences(need_gpu, precision, tol):
         assert abs(a - r) <= tol * abs(r), (name, a, r)
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64", "f32/f32d"])
 def test_deferred_coefficient_sums_equal_immediate(need_gpu, precision):
     """Backward calls with grads = NULL keep the coefficient sums in the
     scratch buffer, also when it is fresh (no zsc_adj_seed before); one
@@ -309,3 +315,51 @@ def test_mixed_adjoint_dot_product_and_refusals(need_gpu):
     for ctx_prec, lam_prec in (("f32", "f64"), ("f32", "bf16"), ("f64", "bf16")):
         with zsc.Context(*dims, precision=ctx_prec) as ctx:
             assert lib().zsc_adj_set_lambda_precision(ctx._h, PRECISIONS[lam_prec]) == ZSC_ERR_PRECISION
+
+
+def test_deviation_state_adjoint_step_equals_fp32(need_gpu):
+    """One adjoint step over an f32d state (g as deviations from w_i rho_mean)
+    against the same step over the FP32 state with the rest added back:
+    lam_out agrees to FP32 rounding, and so do the coefficient sums, except
+    that the deviation form leaves out of inv_tau_rho_one and k0..k2 the rest
+    state's part rho_mean sum_x sum_i w_i lam_i (resp. rho_mean sum over its
+    population group), which cancels in the tau_g derivative; both are
+    checked here, with that part formed in FP64 from the adjoint input."""
+    import torch
+    dims = (12, 10, 8)
+    p, phi0 = small_problem(dims, seed=8)
+    W = torch.tensor([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=torch.float64, device="cuda")
+    rest = (W.float() * np.float32(p.rho_mean)).double()   # w_i rho_mean as the library forms it
+    runs = {}
+    for states in ("f32", "f32d"):
+        runs[states] = adjoint.AdjointRunner(dims, p, precision="f32", states=states,
+                                             checkpoint_every=4)
+    a = runs["f32"].forward(adjoint.theta_of(p), torch.from_numpy(phi0), 3)
+    runs["f32d"]._set_theta(adjoint.theta_of(p))
+    adev = a.clone()
+    adev[7:] = (a[7:].double() - rest[:, None]).float()
+    g = torch.Generator(device="cuda").manual_seed(9)
+    w = torch.randn((26, runs["f32"].n), dtype=torch.float32, device="cuda", generator=g)
+    out = {}
+    for states, st in (("f32", a), ("f32d", adev)):
+        run = runs[states]
+        lam = run._lam()
+        run.grads.zero_()
+        gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
+        torch.cuda.synchronize()
+        check(lib().zsc_adj_backward(run.ctx._h, _vp(st), 0, None, _vp(w), _vp(lam),
+                                     _vp(run.scratch), gp))
+        run.ctx.sync()
+        out[states] = (lam.double(), run.grads.clone())
+    (l32, g32), (ld, gd) = out["f32"], out["f32d"]
+    e_lam = float((l32 - ld).abs().max() / l32.abs().max())
+    wg = w[7:].double().sum(1)                              # sum over cells per g population
+    part = {6: p.rho_mean * float((W * wg).sum()), 7: p.rho_mean * float(wg[0]),
+            8: p.rho_mean * float(wg[1:7].sum()), 9: p.rho_mean * float(wg[7:].sum())}
+    print("lam_out rel diff", e_lam, "\nf32 ", g32[:14].tolist(), "\nf32d", gd[:14].tolist(),
+          "\nrest part", part, flush=True)
+    assert e_lam <= 1e-4
+    for j in range(14):
+        want = float(g32[j]) - part.get(j, 0.0)
+        scale = max(abs(float(g32[j])), abs(part.get(j, 0.0)), 1e-30)
+        assert abs(float(gd[j]) - want) <= 1e-4 * scale, (j, float(gd[j]), want)
