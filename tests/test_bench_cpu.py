@@ -23,8 +23,9 @@ def test_traffic_summary_is_read_and_at_least_algorithmic():
     import bench
     t = bench.traffic_from_profile((512, 512, 512))
     cells = 512 ** 3
-    assert set(t) == {"f32", "bf16", "fp16"}
+    assert set(t) == {"f32", "bf16", "fp16", "f32d"}
     assert t["f32"] >= 208 * cells and t["bf16"] >= 108 * cells and t["fp16"] >= 108 * cells
+    assert t["f32d"] >= 208 * cells
     assert bench.traffic_from_profile((7, 7, 7)) == {}
 
 
