@@ -216,9 +216,10 @@ int zsc_state_set(zsc_ctx *ctx, const double *f_host, const double *g_host);
 int zsc_state_moments(zsc_ctx *ctx, double *phi, double *rho, double *ux,
                       double *uy, double *uz);
 /* Copies the A state into / from reference-style device arrays (A half of
- * each 2N array, the f/g layout of src/main.cu:225-251; F64 and F32
- * contexts only): moves a state between the drop-in kernels and the fused
- * path.  No reference counterpart. */
+ * each 2N array, the f/g layout of src/main.cu:225-251; F64, F32 and F32D
+ * contexts, float arrays for the last two, F32D adding w_i * rho_mean to g
+ * on export and taking it off on import): moves a state between the drop-in
+ * kernels and the fused path.  No reference counterpart. */
 int zsc_state_export(zsc_ctx *ctx, void *const f[7], void *const g[19]);
 int zsc_state_import(zsc_ctx *ctx, void *const f[7], void *const g[19]);
 

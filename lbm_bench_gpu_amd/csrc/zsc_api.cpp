@@ -853,13 +853,23 @@ static int xport(zsc_ctx *c, void *const *f, void *const *g, const char *name)
         if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
         if (int r = ensure_alloc(c)) return r;
     }
-    return dropin(c, false, name, [&](auto tag) {
+    auto go = [&](auto tag, const zsc_params *dev) -> hipError_t {
         using T = ZSC_T;
-        if (Export) return DropIn<T>::export_state(c->geo, c->stream, a, (T *const *)f, (T *const *)g);
-        hipError_t e = DropIn<T>::import_state(c->geo, c->stream, c->lat[c->cur], (T *const *)f, (T *const *)g);
+        if (Export)
+            return DropIn<T>::export_state(c->geo, c->stream, a, (T *const *)f, (T *const *)g, dev);
+        hipError_t e = DropIn<T>::import_state(c->geo, c->stream, c->lat[c->cur], (T *const *)f,
+                                               (T *const *)g, dev);
         if (e == hipSuccess) c->state = STATE_A;
         return e;
-    });
+    };
+    if (c->prec == ZSC_F32D) {
+        // float reference arrays; the lattice's g are deviations from w_i rho_mean
+        if (!c->have_params) return fail(ZSC_ERR_STATE, "%s: zsc_set_params first", name);
+        if (int r = set_device(c)) return r;
+        const hipError_t e = go(Tag<float>{}, &c->params);
+        return e == hipSuccess ? ZSC_OK : hip_fail(e, name);
+    }
+    return dropin(c, false, name, [&](auto tag) { return go(tag, nullptr); });
 }
 
 extern "C" int zsc_state_export(zsc_ctx *c, void *const f[7], void *const g[19])

@@ -1809,9 +1809,17 @@ __global__ void __launch_bounds__(256) k_stream_g(int nx, int ny, int nz, int64_
     }
 }
 
+// The rest values w_i rho_mean of the 19 g populations (kernel argument).
 template <class T>
+struct GRest {
+    T v[19];
+};
+
+// DEV: the lattice's g are deviations from the rest values (f32d), which
+// export adds back and import takes off.
+template <class T, bool DEV>
 __global__ void __launch_bounds__(256)
-    k_export(int64_t n, Lattice<T> A, Pops<T, 7> f, Pops<T, 19> g)
+    k_export(int64_t n, Lattice<T> A, Pops<T, 7> f, Pops<T, 19> g, GRest<T> rest)
 {
     ZSC_GRID_STRIDE(idx, n)
     {
@@ -1819,13 +1827,16 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 1; i < 7; ++i) f.p[i][idx] = A.f[(int64_t)(i - 1) * n + idx];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) g.p[i][idx] = A.g[(int64_t)i * n + idx];
+        for (int i = 0; i < 19; ++i) {
+            const T v = A.g[(int64_t)i * n + idx];
+            g.p[i][idx] = DEV ? v + rest.v[i] : v;
+        }
     }
 }
 
-template <class T>
+template <class T, bool DEV>
 __global__ void __launch_bounds__(256)
-    k_import(int64_t n, Lattice<T> A, Pops<T, 7> f, Pops<T, 19> g)
+    k_import(int64_t n, Lattice<T> A, Pops<T, 7> f, Pops<T, 19> g, GRest<T> rest)
 {
     ZSC_GRID_STRIDE(idx, n)
     {
@@ -1833,8 +1844,23 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 1; i < 7; ++i) A.f[(int64_t)(i - 1) * n + idx] = f.p[i][idx];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) A.g[(int64_t)i * n + idx] = g.p[i][idx];
+        for (int i = 0; i < 19; ++i) {
+            const T v = g.p[i][idx];
+            A.g[(int64_t)i * n + idx] = DEV ? v - rest.v[i] : v;
+        }
     }
+}
+
+// w_i rho_mean as the fused path forms it (g_rest), or zeros.
+template <class T>
+inline GRest<T> grest_of(const zsc_params *p)
+{
+    GRest<T> r{};
+    if (p) {
+        const Coef<T> c = make_coef<T>(*p, 0);
+        for (int i = 0; i < 19; ++i) r.v[i] = g_weight<T>(i) * c.rho_mean;
+    }
+    return r;
 }
 
 #define ZSC_LAUNCH(kern, ...)                                                     \
@@ -1908,17 +1934,20 @@ hipError_t DropIn<T>::stream_g(const Geo &g, hipStream_t s, T *const gp[19])
 
 template <class T>
 hipError_t DropIn<T>::export_state(const Geo &g, hipStream_t s, const void *A, T *const f[7],
-                                   T *const gp[19])
+                                   T *const gp[19], const zsc_params *dev)
 {
-    ZSC_LAUNCH(k_export<T>, g.n, LatLayout<T>::view(const_cast<void *>(A), g.n), pops<T, 7>(f),
-               pops<T, 19>(gp));
+    const Lattice<T> v = LatLayout<T>::view(const_cast<void *>(A), g.n);
+    if (dev) ZSC_LAUNCH((k_export<T, true>), g.n, v, pops<T, 7>(f), pops<T, 19>(gp), grest_of<T>(dev));
+    ZSC_LAUNCH((k_export<T, false>), g.n, v, pops<T, 7>(f), pops<T, 19>(gp), grest_of<T>(nullptr));
 }
 
 template <class T>
 hipError_t DropIn<T>::import_state(const Geo &g, hipStream_t s, void *A, T *const f[7],
-                                   T *const gp[19])
+                                   T *const gp[19], const zsc_params *dev)
 {
-    ZSC_LAUNCH(k_import<T>, g.n, LatLayout<T>::view(A, g.n), pops<T, 7>(f), pops<T, 19>(gp));
+    const Lattice<T> v = LatLayout<T>::view(A, g.n);
+    if (dev) ZSC_LAUNCH((k_import<T, true>), g.n, v, pops<T, 7>(f), pops<T, 19>(gp), grest_of<T>(dev));
+    ZSC_LAUNCH((k_import<T, false>), g.n, v, pops<T, 7>(f), pops<T, 19>(gp), grest_of<T>(nullptr));
 }
 
 #undef ZSC_LAUNCH

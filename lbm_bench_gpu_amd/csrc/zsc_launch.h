@@ -114,11 +114,12 @@ struct DropIn {
                                   const T *uz, T *const g[19]);
     static hipError_t stream_f(const Geo &, const zsc_params &, hipStream_t, T *const f[7]);
     static hipError_t stream_g(const Geo &, hipStream_t, T *const g[19]);
-    // A half of reference arrays <-> context lattice A.
+    // A half of reference arrays <-> context lattice A; dev (non-null): the
+    // lattice keeps g as deviations from w_i rho_mean of these parameters.
     static hipError_t export_state(const Geo &, hipStream_t, const void *A, T *const f[7],
-                                   T *const g[19]);
+                                   T *const g[19], const zsc_params *dev = nullptr);
     static hipError_t import_state(const Geo &, hipStream_t, void *A, T *const f[7],
-                                   T *const g[19]);
+                                   T *const g[19], const zsc_params *dev = nullptr);
 };
 
 // Reverse mode of the step on plain (26, n) lattices (zsc_adjoint.h): states

@@ -47,6 +47,8 @@ def test_dropin_and_adjoint_need_f32_or_f64(need_gpu):
             ctx.set_params(p)
             assert _code(ctx.init_f, phi, f) == ZSC_ERR_PRECISION
             assert _code(ctx.update_phi, phi, f) == ZSC_ERR_PRECISION
+            if prec == "f32d":
+                continue  # the adjoint runs over f32d states (tests/test_gpu_adjoint.py)
             buf = torch.zeros(26 * n, dtype=torch.float32, device="cuda")
             rc = lib().zsc_adj_init(ctx._h, ctypes.c_void_p(phi.data_ptr()),
                                     ctypes.c_void_p(buf.data_ptr()))

@@ -329,6 +329,52 @@ def test_moment_stats_on_dropin_arrays(need_gpu, prec):
         assert s == s_state
 
 
+def test_f32d_state_export_and_import(need_gpu):
+    """An f32d context (g stored as deviations from w_i rho_mean) exports the
+    reference's float arrays: f as held, g with the rest value added back,
+    equal to the context's own populations rounded to FP32, and within FP32
+    rounding of an FP32 context's export after the same steps; importing
+    those arrays into a new f32d context reproduces the populations to
+    FP32 rounding, and both contexts step on alike."""
+    import torch
+    dims = (64, 12, 10)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    p, phi0 = small_case(dims, seed=29)
+    out = {}
+    for prec in ("f32", "f32d"):
+        f, g, _ = _dropin_tensors(n, torch.float32)
+        with zsc.Context(*dims, precision=prec) as ctx:
+            ctx.set_params(p)
+            ctx.state_init(phi0)
+            ctx.step(5)
+            ctx.state_export(f, g)
+            held = ctx.state_get()
+            ctx.step(3)
+            after = ctx.state_get()
+            ctx.sync()
+        fa = np.stack([t[:n].double().cpu().numpy() for t in f]).reshape(7, nz, ny, nx)
+        ga = np.stack([t[:n].double().cpu().numpy() for t in g]).reshape(19, nz, ny, nx)
+        out[prec] = (f, g, fa, ga, held, after)
+    _, _, fa, ga, held, after = out["f32d"]
+    assert np.array_equal(fa, held[0].astype(np.float32).astype(np.float64))
+    assert np.array_equal(ga, held[1].astype(np.float32).astype(np.float64))
+    e32 = max(rel(fa, out["f32"][2]), rel(ga, out["f32"][3]))
+    print("f32d export vs f32 export", e32, flush=True)
+    assert e32 <= 1e-5
+    f, g = out["f32d"][0], out["f32d"][1]
+    with zsc.Context(*dims, precision="f32d") as ctx:
+        ctx.set_params(p)
+        ctx.state_import(f, g)
+        back = ctx.state_get()
+        ctx.step(3)
+        stepped = ctx.state_get()
+    e_back = max(rel(back[0], held[0]), rel(back[1], held[1]))
+    e_step = max(rel(stepped[0], after[0]), rel(stepped[1], after[1]))
+    print("f32d import round trip", e_back, "after 3 steps", e_step, flush=True)
+    assert e_back <= 1e-7 and e_step <= 1e-6
+
+
 STATS_NAMES = ["sum_phi", "sum_rho", "bubble_volume", "ux_sum", "uy_sum", "uz_sum", "umax_sq",
                "p_in_sum", "p_out_sum", "bubble_volume_full"]
 
