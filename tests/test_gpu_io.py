@@ -83,7 +83,7 @@ def test_vtk_ascii_matches_reference_layout(need_gpu, tmp_path):
     assert not bad, [(i, lines[i], expect[i]) for i in bad[:5]]
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("prec", ["f32", "f32d", "bf16"])
 def test_vtk_binary_holds_the_fields(need_gpu, tmp_path, prec):
     dims = (16, 12, 9)
     nx, ny, nz = dims

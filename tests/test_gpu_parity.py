@@ -526,7 +526,7 @@ def test_fp32_at_the_offset_limit(need_gpu):
         assert e.value.code in (ZSC_ERR_ARG,)
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32d", "bf16"])
 def test_placed_lattices_give_the_same_steps(need_gpu, prec):
     """zsc_place_lattices only chooses which allocation holds the lattices:
     the state set and stepped afterwards equals an unplaced context's bit for
