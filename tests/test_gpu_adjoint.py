@@ -312,7 +312,8 @@ def test_mixed_adjoint_dot_product_and_refusals(need_gpu):
     rhs = float((v * lam.double()).sum())
     print("mixed dot product", lhs, rhs, flush=True)
     assert abs(lhs - rhs) <= 1e-6 * abs(rhs)
-    for ctx_prec, lam_prec in (("f32", "f64"), ("f32", "bf16"), ("f64", "bf16")):
+    for ctx_prec, lam_prec in (("f32", "f64"), ("f32", "bf16"), ("f64", "bf16"), ("f32d", "f64"),
+                               ("f32d", "bf16")):
         with zsc.Context(*dims, precision=ctx_prec) as ctx:
             assert lib().zsc_adj_set_lambda_precision(ctx._h, PRECISIONS[lam_prec]) == ZSC_ERR_PRECISION
 
