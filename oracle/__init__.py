@@ -8,6 +8,10 @@ It wraps ``liboracle.so`` (built from ``zsc_oracle.c`` by ``oracle/Makefile``),
 an FP64 C restatement of the reference kernels (citations in the C source), and
 adds NumPy helpers.  Arrays are NumPy float64, populations pop-major with shape
 ``(q, nz, ny, nx)`` -- flat index ``x + nx*(y + ny*z)`` per population.
+``liboracle_f32.so`` is the same source evaluated in FP32 (the reference's
+kernels compute in float): ``State(..., dtype=np.float32)`` runs the step
+with float32 arrays, FP32 arithmetic and the coefficients rounded to FP32;
+it is pinned against the FP64 restatement (agreement to FP32 rounding).
 
 Pinning of each function (the checks live in ``tests/test_oracle.py``):
   zo_derive           ZSC identities sigma = (4/3) sqrt(2 kappa beta) phi*^3 and
@@ -38,6 +42,8 @@ Pinning of each function (the checks live in ``tests/test_oracle.py``):
                       square-gradient terms on Fourier modes of phi (closed
                       forms from the stencil symbols); the pressure sums of
                       zo_stats.
+  FP32 evaluation     within FP32 rounding of the FP64 restatement, and not
+                      equal to it (tests/test_oracle.py).
 """
 from __future__ import annotations
 
@@ -49,7 +55,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_LIB32_PATH = os.path.join(_HERE, "liboracle_f32.so")
 _lib = None
+_lib32 = None
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -67,6 +75,31 @@ def build() -> None:
     """Compile liboracle.so with make (called by the repo's build step)."""
     import subprocess
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _declare(L, P):
+    """Argument types of the step functions, arrays of pointer type P."""
+    L.zo_init_f.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef), P, P]
+    L.zo_init_g.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef), P, P, P]
+    L.zo_update_phi.argtypes = [_I, _I, _I, P, P]
+    L.zo_update_rho.argtypes = [_I, _I, _I, P, P]
+    L.zo_update_velocity.argtypes = [_I, _I, _I, P, P, P, P, P]
+    L.zo_run.argtypes = [_I, _I, _I, ctypes.POINTER(ZoCoef), _I, _I, P, P, P]
+    for f in ("zo_init_f", "zo_init_g", "zo_update_phi", "zo_update_rho", "zo_update_velocity",
+              "zo_run"):
+        getattr(L, f).restype = None
+
+
+def lib32():
+    """The FP32 evaluation of the restatement (float32 arrays)."""
+    global _lib32
+    if _lib32 is None:
+        if not os.path.exists(_LIB32_PATH):
+            build()
+        L = ctypes.CDLL(_LIB32_PATH)
+        _declare(L, ctypes.POINTER(ctypes.c_float))
+        _lib32 = L
+    return _lib32
 
 
 def lib():
@@ -134,6 +167,20 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(_P)
 
 
+def _p32(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _step_lib(dtype):
+    """(library, pointer helper) of a lattice dtype: float64 or float32."""
+    if np.dtype(dtype) == np.float64:
+        return lib(), _p
+    if np.dtype(dtype) == np.float32:
+        return lib32(), _p32
+    raise TypeError(f"oracle lattices are float64 or float32, not {dtype}")
+
+
 @dataclass
 class Physics:
     """Raw inputs of Input/properties.in plus the overrides the configs need."""
@@ -166,32 +213,36 @@ def init_phi(shape, bubbles, phi_star, width) -> np.ndarray:
 
 
 class State:
-    """A lattice (post-stream, reference convention) of one periodic box."""
+    """A lattice (post-stream, reference convention) of one periodic box;
+    dtype float64 (the FP64 oracle) or float32 (its FP32 evaluation)."""
 
-    def __init__(self, shape, coef: ZoCoef):
+    def __init__(self, shape, coef: ZoCoef, dtype=np.float64):
         self.nz, self.ny, self.nx = shape
         self.c = coef
-        self.f = np.zeros((7,) + tuple(shape))
-        self.g = np.zeros((19,) + tuple(shape))
+        self.dtype = np.dtype(dtype)
+        self._L, self._pp = _step_lib(self.dtype)
+        self.f = np.zeros((7,) + tuple(shape), dtype=self.dtype)
+        self.g = np.zeros((19,) + tuple(shape), dtype=self.dtype)
         n = self.nx * self.ny * self.nz
-        self._work = np.empty(31 * n)
+        self._work = np.empty(31 * n, dtype=self.dtype)
 
     @property
     def dims(self):
         return self.nx, self.ny, self.nz
 
     def init(self, phi: np.ndarray, rho: np.ndarray | None = None):
-        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        L, pp = self._L, self._pp
+        phi = np.ascontiguousarray(phi, dtype=self.dtype)
         if rho is None:
             rho = np.full_like(phi, self.c.rho_mean)
-        rho = np.ascontiguousarray(rho, dtype=np.float64)
-        lib().zo_init_f(*self.dims, ctypes.byref(self.c), _p(phi), _p(self.f))
-        lib().zo_init_g(*self.dims, ctypes.byref(self.c), _p(phi), _p(rho), _p(self.g))
+        rho = np.ascontiguousarray(rho, dtype=self.dtype)
+        L.zo_init_f(*self.dims, ctypes.byref(self.c), pp(phi), pp(self.f))
+        L.zo_init_g(*self.dims, ctypes.byref(self.c), pp(phi), pp(rho), pp(self.g))
         return self
 
     def run(self, nsteps: int, grav_on: bool = True):
-        lib().zo_run(*self.dims, ctypes.byref(self.c), int(grav_on), int(nsteps),
-                     _p(self.f), _p(self.g), _p(self._work))
+        self._L.zo_run(*self.dims, ctypes.byref(self.c), int(grav_on), int(nsteps),
+                       self._pp(self.f), self._pp(self.g), self._pp(self._work))
         return self
 
     def moments(self):
@@ -199,17 +250,18 @@ class State:
 
 
 def moments(f: np.ndarray, g: np.ndarray):
-    """phi, rho, ux, uy, uz of an A lattice (src/update_*.cu)."""
+    """phi, rho, ux, uy, uz of an A lattice (src/update_*.cu), at the
+    lattice's precision."""
     shape = f.shape[1:]
     nz, ny, nx = shape
-    phi, rho = np.empty(shape), np.empty(shape)
-    ux, uy, uz = np.empty(shape), np.empty(shape), np.empty(shape)
+    L, pp = _step_lib(f.dtype)
+    phi, rho = np.empty(shape, f.dtype), np.empty(shape, f.dtype)
+    ux, uy, uz = np.empty(shape, f.dtype), np.empty(shape, f.dtype), np.empty(shape, f.dtype)
     f = np.ascontiguousarray(f)
-    g = np.ascontiguousarray(g)
-    L = lib()
-    L.zo_update_phi(nx, ny, nz, _p(f), _p(phi))
-    L.zo_update_rho(nx, ny, nz, _p(g), _p(rho))
-    L.zo_update_velocity(nx, ny, nz, _p(g), _p(rho), _p(ux), _p(uy), _p(uz))
+    g = np.ascontiguousarray(g, dtype=f.dtype)
+    L.zo_update_phi(nx, ny, nz, pp(f), pp(phi))
+    L.zo_update_rho(nx, ny, nz, pp(g), pp(rho))
+    L.zo_update_velocity(nx, ny, nz, pp(g), pp(rho), pp(ux), pp(uy), pp(uz))
     return phi, rho, ux, uy, uz
 
 

@@ -26,14 +26,23 @@ static const int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 
 static const int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
 static const int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
 
-/* Weights of src/constants.h:43-59, at FP64. */
-#define INV6 (1.0 / 6.0)
-#define INV12 (1.0 / 12.0)
-#define INV_CS_SQ 3.0
-#define CS_SQ (1.0 / 3.0)
-#define W0 (1.0 / 3.0)
-#define W1 (1.0 / 18.0)
-#define W2 (1.0 / 36.0)
+/* The precision of the step's arithmetic and arrays: double (the FP64
+ * oracle, liboracle.so) or float (liboracle_f32.so, built with
+ * -DZO_REAL=float: the same expressions evaluated in FP32, as the reference
+ * evaluates them, with every constant and coefficient rounded to FP32 first).
+ * The parameter derivation, the initial profile and the statistics' sums
+ * stay FP64 in both. */
+typedef zo_real real;
+#define RL(x) ((real)(x))
+
+/* Weights of src/constants.h:43-59 at the step's precision. */
+#define INV6 RL(1.0 / 6.0)
+#define INV12 RL(1.0 / 12.0)
+#define INV_CS_SQ RL(3.0)
+#define CS_SQ RL(1.0 / 3.0)
+#define W0 RL(1.0 / 3.0)
+#define W1 RL(1.0 / 18.0)
+#define W2 RL(1.0 / 36.0)
 #define WC0 1.0
 #define WC1 (1.0 / 6.0)
 #define WC2 (1.0 / 12.0)
@@ -54,8 +63,8 @@ static inline size_t cell(int nx, int ny, int x, int y, int z)
 }
 
 /* p[d] = phi(x + c_d) for the 19 g directions, periodic. */
-static inline void gather19(int nx, int ny, int nz, const double *phi, int x,
-                            int y, int z, double p[19])
+static inline void gather19(int nx, int ny, int nz, const real *phi, int x,
+                            int y, int z, real p[19])
 {
     for (int d = 0; d < 19; ++d)
         p[d] = phi[cell(nx, ny, wrap(x + CX[d], nx), wrap(y + CY[d], ny),
@@ -64,28 +73,28 @@ static inline void gather19(int nx, int ny, int nz, const double *phi, int x,
 
 /* Laplacian of src/collision_f.cu:61-70 (same in init_f/init_g/collision_g),
  * with phi(ie,jn,k)=p7, phi(iw,js,k)=p8, ... mapped to g directions. */
-static inline double lap19(const double p[19])
+static inline real lap19(const real p[19])
 {
     return (p[7] + p[8] + p[9] + p[10] + p[11] + p[12] + p[13] + p[14] + p[15] +
             p[16] + p[17] + p[18] +
-            2.0 * (p[1] + p[2] + p[3] + p[4] + p[5] + p[6] - 12.0 * p[0])) *
+            RL(2.0) * (p[1] + p[2] + p[3] + p[4] + p[5] + p[6] - RL(12.0) * p[0])) *
            INV6;
 }
 
 /* Isotropic gradient bracket of src/collision_f.cu:78-94 (before mu*, *inv12). */
-static inline double gx19(const double p[19])
+static inline real gx19(const real p[19])
 {
-    return 2.0 * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] +
+    return RL(2.0) * (p[1] - p[2]) + p[7] - p[8] + p[9] - p[10] + p[11] - p[12] +
            p[13] - p[14];
 }
-static inline double gy19(const double p[19])
+static inline real gy19(const real p[19])
 {
-    return 2.0 * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] +
+    return RL(2.0) * (p[3] - p[4]) + p[7] - p[8] + p[10] - p[9] + p[15] - p[16] +
            p[17] - p[18];
 }
-static inline double gz19(const double p[19])
+static inline real gz19(const real p[19])
 {
-    return 2.0 * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] +
+    return RL(2.0) * (p[5] - p[6]) + p[11] - p[12] + p[14] - p[13] + p[15] - p[16] +
            p[18] - p[17];
 }
 
@@ -148,8 +157,8 @@ void zo_init_phi(int nx, int ny, int nz, const double *bubbles, int nb,
             }
 }
 
-void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-               double *f)
+void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const real *phi,
+               real *f)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/init_f.cu:42-79 */
@@ -158,19 +167,19 @@ void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const double *phi,
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 size_t idx = cell(nx, ny, x, y, z);
-                double p[19];
+                real p[19];
                 gather19(nx, ny, nz, phi, x, y, z, p);
-                double phin = p[0];
-                double lap = lap19(p);
-                double mu = c->alpha4 * phin * (phin * phin - c->phi_star_sq) - c->kappa * lap;
-                double af = g_mut == ZO_MUT_INITF_AF_NO_HALF ? c->gamma * mu : 0.5 * c->gamma * mu;
-                f[idx] = phin - 3.0 * c->gamma * mu;
+                real phin = p[0];
+                real lap = lap19(p);
+                real mu = RL(c->alpha4) * phin * (phin * phin - RL(c->phi_star_sq)) - RL(c->kappa) * lap;
+                real af = g_mut == ZO_MUT_INITF_AF_NO_HALF ? RL(c->gamma) * mu : RL(0.5) * RL(c->gamma) * mu;
+                f[idx] = phin - RL(3.0) * RL(c->gamma) * mu;
                 for (int i = 1; i < 7; ++i) f[i * n + idx] = af;
             }
 }
 
-void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-               const double *rho, double *g)
+void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const real *phi,
+               const real *rho, real *g)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/init_g.cu:46-93 */
@@ -179,23 +188,23 @@ void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const double *phi,
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 size_t idx = cell(nx, ny, x, y, z);
-                double p[19];
+                real p[19];
                 gather19(nx, ny, nz, phi, x, y, z, p);
-                double phin = p[0];
-                double rhon = rho[idx];
-                double lap = lap19(p);
-                double mu = c->alpha4 * phin * (phin * phin - c->phi_star_sq) - c->kappa * lap;
-                double ag1 = g_mut == ZO_MUT_INITG_AXIS_PHIMU ? W1 * (phin * mu + rhon)
-                                                              : W1 * (3.0 * phin * mu + rhon);
-                double ag2 = W2 * (3.0 * phin * mu + rhon);
-                g[idx] = g_mut == ZO_MUT_INITG_REST_3PHIMU ? W0 * (rhon - 3.0 * phin * mu)
-                                                           : W0 * (rhon - 6.0 * phin * mu);
+                real phin = p[0];
+                real rhon = rho[idx];
+                real lap = lap19(p);
+                real mu = RL(c->alpha4) * phin * (phin * phin - RL(c->phi_star_sq)) - RL(c->kappa) * lap;
+                real ag1 = g_mut == ZO_MUT_INITG_AXIS_PHIMU ? W1 * (phin * mu + rhon)
+                                                              : W1 * (RL(3.0) * phin * mu + rhon);
+                real ag2 = W2 * (RL(3.0) * phin * mu + rhon);
+                g[idx] = g_mut == ZO_MUT_INITG_REST_3PHIMU ? W0 * (rhon - RL(3.0) * phin * mu)
+                                                           : W0 * (rhon - RL(6.0) * phin * mu);
                 for (int i = 1; i < 7; ++i) g[i * n + idx] = ag1;
                 for (int i = 7; i < 19; ++i) g[i * n + idx] = ag2;
             }
 }
 
-void zo_update_phi(int nx, int ny, int nz, const double *f, double *phi)
+void zo_update_phi(int nx, int ny, int nz, const real *f, real *phi)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/update_phi.cu:40-44 */
@@ -205,27 +214,27 @@ void zo_update_phi(int nx, int ny, int nz, const double *f, double *phi)
                    f[4 * n + idx] + f[5 * n + idx] + f[6 * n + idx];
 }
 
-void zo_update_rho(int nx, int ny, int nz, const double *g, double *rho)
+void zo_update_rho(int nx, int ny, int nz, const real *g, real *rho)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/update_rho.cu:44-51: left-to-right sum g0 + ... + g18 */
 #pragma omp parallel for schedule(static)
     for (size_t idx = 0; idx < n; ++idx) {
-        double s = g[idx];
+        real s = g[idx];
         for (int i = 1; i < 19; ++i) s = s + g[i * n + idx];
         rho[idx] = s;
     }
 }
 
-void zo_update_velocity(int nx, int ny, int nz, const double *g,
-                        const double *rho, double *ux, double *uy, double *uz)
+void zo_update_velocity(int nx, int ny, int nz, const real *g,
+                        const real *rho, real *ux, real *uy, real *uz)
 {
     const size_t n = (size_t)nx * ny * nz;
 #define G(i) g[(size_t)(i) * n + idx]
     /* src/update_velocity.cu:45-61 */
 #pragma omp parallel for schedule(static)
     for (size_t idx = 0; idx < n; ++idx) {
-        double inv_rho = 1.0 / rho[idx];
+        real inv_rho = RL(1.0) / rho[idx];
         ux[idx] = (G(1) - G(2) + G(7) - G(8) + G(9) - G(10) + G(11) - G(12) +
                    G(13) - G(14)) * inv_rho;
         uy[idx] = (G(3) - G(4) + G(7) - G(8) - G(9) + G(10) + G(15) - G(16) +
@@ -237,24 +246,24 @@ void zo_update_velocity(int nx, int ny, int nz, const double *g,
 }
 
 /* Chemical potential and force of src/collision_f.cu:61-96. */
-static inline void force(const zo_coef *c, int grav_on, const double p[19],
-                         double *mu, double *fx, double *fy, double *fz)
+static inline void force(const zo_coef *c, int grav_on, const real p[19],
+                         real *mu, real *fx, real *fy, real *fz)
 {
-    double phin = p[0];
-    double lap = lap19(p);
-    double m = c->alpha4 * phin * (phin * phin - c->phi_star_sq) - c->kappa * lap;
+    real phin = p[0];
+    real lap = lap19(p);
+    real m = RL(c->alpha4) * phin * (phin * phin - RL(c->phi_star_sq)) - RL(c->kappa) * lap;
     *mu = m;
     *fx = m * gx19(p) * INV12;
     *fy = m * gy19(p) * INV12;
-    double z = m * gz19(p) * INV12;
-    if (phin > 0.0) z = z + (grav_on ? c->grav : 0.0);
+    real z = m * gz19(p) * INV12;
+    if (phin > RL(0.0)) z = z + (grav_on ? RL(c->grav) : RL(0.0));
     *fz = z;
 }
 
 void zo_collision_f(int nx, int ny, int nz, const zo_coef *c, int grav_on,
-                    const double *phi, const double *rho, const double *ux,
-                    const double *uy, const double *uz, const double *fa,
-                    double *fb)
+                    const real *phi, const real *rho, const real *ux,
+                    const real *uy, const real *uz, const real *fa,
+                    real *fb)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/collision_f.cu:44-116 */
@@ -263,22 +272,22 @@ void zo_collision_f(int nx, int ny, int nz, const zo_coef *c, int grav_on,
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 size_t idx = cell(nx, ny, x, y, z);
-                double p[19], mu, fx, fy, fz;
+                real p[19], mu, fx, fy, fz;
                 gather19(nx, ny, nz, phi, x, y, z, p);
-                double phin = p[0];
-                double inv_rho = 1.0 / rho[idx];
+                real phin = p[0];
+                real inv_rho = RL(1.0) / rho[idx];
                 force(c, grav_on, p, &mu, &fx, &fy, &fz);
-                double uxn = ux[idx] + 0.5 * fx * inv_rho;
-                double uyn = uy[idx] + 0.5 * fy * inv_rho;
-                double uzn = uz[idx] + 0.5 * fz * inv_rho;
-                double af = 0.5 * c->gamma * c->inv_tau_phi * mu;
-                double cf = c->inv_tau_phi * c->inv_eta2 * phin;
-                double t = c->inv_tau_phi_one;
-                if (g_mut == ZO_MUT_AF_NO_TAU) af = 0.5 * c->gamma * mu;
-                if (g_mut == ZO_MUT_CF_NO_TAU) cf = c->inv_eta2 * phin;
-                if (g_mut == ZO_MUT_CF_HALF) cf = c->inv_tau_phi * 0.5 * phin;
-                fb[idx] = g_mut == ZO_MUT_F0_ONE_AF ? t * fa[idx] - af + c->inv_tau_phi * phin
-                                                    : t * fa[idx] - 6.0 * af + c->inv_tau_phi * phin;
+                real uxn = ux[idx] + RL(0.5) * fx * inv_rho;
+                real uyn = uy[idx] + RL(0.5) * fy * inv_rho;
+                real uzn = uz[idx] + RL(0.5) * fz * inv_rho;
+                real af = RL(0.5) * RL(c->gamma) * RL(c->inv_tau_phi) * mu;
+                real cf = RL(c->inv_tau_phi) * RL(c->inv_eta2) * phin;
+                real t = RL(c->inv_tau_phi_one);
+                if (g_mut == ZO_MUT_AF_NO_TAU) af = RL(0.5) * RL(c->gamma) * mu;
+                if (g_mut == ZO_MUT_CF_NO_TAU) cf = RL(c->inv_eta2) * phin;
+                if (g_mut == ZO_MUT_CF_HALF) cf = RL(c->inv_tau_phi) * RL(0.5) * phin;
+                fb[idx] = g_mut == ZO_MUT_F0_ONE_AF ? t * fa[idx] - af + RL(c->inv_tau_phi) * phin
+                                                    : t * fa[idx] - RL(6.0) * af + RL(c->inv_tau_phi) * phin;
                 fb[n + idx] = t * fa[n + idx] + af + cf * uxn;
                 fb[2 * n + idx] = t * fa[2 * n + idx] + af - cf * uxn;
                 fb[3 * n + idx] = t * fa[3 * n + idx] + af + cf * uyn;
@@ -289,13 +298,13 @@ void zo_collision_f(int nx, int ny, int nz, const zo_coef *c, int grav_on,
 }
 
 void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
-                    const double *phi, const double *rho, const double *ux,
-                    const double *uy, const double *uz, const double *ga,
-                    double *gb)
+                    const real *phi, const real *rho, const real *ux,
+                    const real *uy, const real *uz, const real *ga,
+                    real *gb)
 {
     const size_t n = (size_t)nx * ny * nz;
-    const double K1 = c->k1, K2 = c->k2, KC1 = c->kc1, KC2 = c->kc2;
-    const double t = c->inv_tau_rho_one;
+    const real K1 = RL(c->k1), K2 = RL(c->k2), KC1 = RL(c->kc1), KC2 = RL(c->kc2);
+    const real t = RL(c->inv_tau_rho_one);
     /* src/collision_g.cu:49-196 */
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < nz; ++z)
@@ -304,49 +313,49 @@ void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
                 size_t idx = cell(nx, ny, x, y, z);
 #define GA(i) ga[(size_t)(i) * n + idx]
 #define GB(i) gb[(size_t)(i) * n + idx]
-                double p[19], mu, fx, fy, fz;
+                real p[19], mu, fx, fy, fz;
                 gather19(nx, ny, nz, phi, x, y, z, p);
-                double phin = p[0];
-                double rhon = rho[idx];
+                real phin = p[0];
+                real rhon = rho[idx];
                 force(c, grav_on, p, &mu, &fx, &fy, &fz);
-                double uxn = ux[idx] + 0.5 * fx / rhon;
-                double uyn = uy[idx] + 0.5 * fy / rhon;
-                double uzn = uz[idx] + 0.5 * fz / rhon;
-                double temp = K1 * (3.0 * phin * mu +
-                                    rhon * (1.0 - 1.5 * (uxn * uxn + uyn * uyn + uzn * uzn)));
-                double uf = uxn * fx + uyn * fy + uzn * fz;
+                real uxn = ux[idx] + RL(0.5) * fx / rhon;
+                real uyn = uy[idx] + RL(0.5) * fy / rhon;
+                real uzn = uz[idx] + RL(0.5) * fz / rhon;
+                real temp = K1 * (RL(3.0) * phin * mu +
+                                    rhon * (RL(1.0) - RL(1.5) * (uxn * uxn + uyn * uyn + uzn * uzn)));
+                real uf = uxn * fx + uyn * fy + uzn * fz;
                 uxn = uxn * INV_CS_SQ;
                 uyn = uyn * INV_CS_SQ;
                 uzn = uzn * INV_CS_SQ;
-                double eq1, eq2;
+                real eq1, eq2;
 
                 GB(0) = t * GA(0) +
-                        c->k0 * (rhon - (g_mut == ZO_MUT_CG_REST_3PHIMU ? 3.0 : 6.0) * phin * mu -
-                                 rhon * (uxn * uxn + uyn * uyn + uzn * uzn) / 6.0) -
-                        c->kc0 * uf;
+                        RL(c->k0) * (rhon - (g_mut == ZO_MUT_CG_REST_3PHIMU ? RL(3.0) : RL(6.0)) * phin * mu -
+                                 rhon * (uxn * uxn + uyn * uyn + uzn * uzn) / RL(6.0)) -
+                        RL(c->kc0) * uf;
 
                 /* axis pairs: note the reference's K1*rhon*u*u*0.5 (x) versus
                  * K1*rhon*u*0.5*u (y, z) operand order, and eq2 = u*(K1 rho + KC1 F) */
-                eq1 = temp + K1 * rhon * uxn * uxn * 0.5 + KC1 * (uxn * fx - uf);
+                eq1 = temp + K1 * rhon * uxn * uxn * RL(0.5) + KC1 * (uxn * fx - uf);
                 eq2 = uxn * (K1 * rhon + KC1 * fx);
                 GB(1) = t * GA(1) + eq1 + eq2;
                 GB(2) = t * GA(2) + eq1 - eq2;
 
-                eq1 = temp + K1 * rhon * uyn * 0.5 * uyn + KC1 * (uyn * fy - uf);
+                eq1 = temp + K1 * rhon * uyn * RL(0.5) * uyn + KC1 * (uyn * fy - uf);
                 eq2 = uyn * (K1 * rhon + KC1 * fy);
                 GB(3) = t * GA(3) + eq1 + eq2;
                 GB(4) = t * GA(4) + eq1 - eq2;
 
-                eq1 = temp + K1 * rhon * uzn * 0.5 * uzn + KC1 * (uzn * fz - uf);
+                eq1 = temp + K1 * rhon * uzn * RL(0.5) * uzn + KC1 * (uzn * fz - uf);
                 eq2 = uzn * (K1 * rhon + KC1 * fz);
                 GB(5) = t * GA(5) + eq1 + eq2;
                 GB(6) = t * GA(6) + eq1 - eq2;
 
-                temp = K2 * ((g_mut == ZO_MUT_CG_DIAG_6PHIMU ? 6.0 : 3.0) * phin * mu +
-                             rhon * (1.0 - (uxn * uxn + uyn * uyn + uzn * uzn) / 6.0));
+                temp = K2 * ((g_mut == ZO_MUT_CG_DIAG_6PHIMU ? RL(6.0) : RL(3.0)) * phin * mu +
+                             rhon * (RL(1.0) - (uxn * uxn + uyn * uyn + uzn * uzn) / RL(6.0)));
 
 #define DIAG(ip, im, s, S)                                                   \
-    eq1 = temp + K2 * rhon * (s) * 0.5 * (s) + KC2 * ((s) * (S) - uf);        \
+    eq1 = temp + K2 * rhon * (s) * RL(0.5) * (s) + KC2 * ((s) * (S) - uf);        \
     eq2 = K2 * rhon * (s) + KC2 * (S);                                        \
     GB(ip) = t * GA(ip) + eq1 + eq2;                                          \
     GB(im) = t * GA(im) + eq1 - eq2;
@@ -363,8 +372,8 @@ void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
             }
 }
 
-void zo_stream_f(int nx, int ny, int nz, const zo_coef *c, const double *fb,
-                 double *fa)
+void zo_stream_f(int nx, int ny, int nz, const zo_coef *c, const real *fb,
+                 real *fa)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/stream_f.cu:52-59 restated as a pull:
@@ -378,12 +387,12 @@ void zo_stream_f(int nx, int ny, int nz, const zo_coef *c, const double *fb,
                 for (int i = 1; i < 7; ++i) {
                     size_t src = cell(nx, ny, wrap(x - CX[i], nx), wrap(y - CY[i], ny),
                                       wrap(z - CZ[i], nz));
-                    fa[i * n + idx] = c->eta * fb[i * n + src] + c->eta2 * fb[i * n + idx];
+                    fa[i * n + idx] = RL(c->eta) * fb[i * n + src] + RL(c->eta2) * fb[i * n + idx];
                 }
             }
 }
 
-void zo_stream_g(int nx, int ny, int nz, const double *gb, double *ga)
+void zo_stream_g(int nx, int ny, int nz, const real *gb, real *ga)
 {
     const size_t n = (size_t)nx * ny * nz;
     /* src/stream_g.cu:56-76 restated as a pull: ga_i(y) = gb_i(y - c_i) */
@@ -401,11 +410,11 @@ void zo_stream_g(int nx, int ny, int nz, const double *gb, double *ga)
 }
 
 void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
-            double *f, double *g, double *work)
+            real *f, real *g, real *work)
 {
     const size_t n = (size_t)nx * ny * nz;
-    double *fb = work, *gb = work + 7 * n;
-    double *phi = gb + 19 * n, *rho = phi + n, *ux = rho + n, *uy = ux + n, *uz = uy + n;
+    real *fb = work, *gb = work + 7 * n;
+    real *phi = gb + 19 * n, *rho = phi + n, *ux = rho + n, *uy = ux + n, *uz = uy + n;
     /* src/main.cu:498-566 with the halo exchanges folded into the wrap */
     for (int s = 0; s < nsteps; ++s) {
         zo_update_phi(nx, ny, nz, f, phi);
@@ -421,34 +430,34 @@ void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
 /* Pressure of src/vtksave.c:75-119 (src/logsave.c:84-137 is the same
  * expression): alpha(phi^2(3phi^2 - 2phi*^2) - phi*^4)
  * - kappa(phi lap - |grad phi|^2/2) + Cs_sq rho, periodic stencil. */
-static inline double pressure_at(const zo_coef *c, const double p[19], double rho)
+static inline real pressure_at(const zo_coef *c, const real p[19], real rho)
 {
-    double phin = p[0], phin2 = phin * phin;
-    double lap = lap19(p);
-    double gx = gx19(p) * INV12, gy = gy19(p) * INV12, gz = gz19(p) * INV12;
-    double gsq = gx * gx + gy * gy + gz * gz;
-    return c->alpha * (phin2 * (3.0 * phin2 - 2.0 * c->phi_star_sq) -
-                       c->phi_star_sq * c->phi_star_sq) -
-           c->kappa * (phin * lap - 0.5 * gsq) + CS_SQ * rho;
+    real phin = p[0], phin2 = phin * phin;
+    real lap = lap19(p);
+    real gx = gx19(p) * INV12, gy = gy19(p) * INV12, gz = gz19(p) * INV12;
+    real gsq = gx * gx + gy * gy + gz * gz;
+    return RL(c->alpha) * (phin2 * (RL(3.0) * phin2 - RL(2.0) * RL(c->phi_star_sq)) -
+                       RL(c->phi_star_sq) * RL(c->phi_star_sq)) -
+           RL(c->kappa) * (phin * lap - RL(0.5) * gsq) + CS_SQ * rho;
 }
 
-void zo_pressure(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-                 const double *rho, double *p_out)
+void zo_pressure(int nx, int ny, int nz, const zo_coef *c, const real *phi,
+                 const real *rho, real *p_out)
 {
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < nz; ++z)
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 size_t idx = cell(nx, ny, x, y, z);
-                double p[19];
+                real p[19];
                 gather19(nx, ny, nz, phi, x, y, z, p);
                 p_out[idx] = pressure_at(c, p, rho[idx]);
             }
 }
 
-void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-              const double *rho, const double *ux, const double *uy,
-              const double *uz, double *out)
+void zo_stats(int nx, int ny, int nz, const zo_coef *c, const real *phi,
+              const real *rho, const real *ux, const real *uy,
+              const real *uz, double *out)
 {
     double s_phi = 0, s_rho = 0, vol = 0, sx = 0, sy = 0, sz = 0, umax = 0;
     double p_in = 0, p_out = 0, vol_full = 0;
@@ -459,16 +468,17 @@ void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
                 size_t idx = cell(nx, ny, x, y, z);
                 s_phi += phi[idx];
                 s_rho += rho[idx];
-                if (phi[idx] >= 0.0) vol_full += 1.0;
+                if (phi[idx] >= RL(0.0)) vol_full += 1.0;
                 /* src/stats.c:39-41 skip the first and last x and y rows */
                 if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2) continue;
-                double usq = ux[idx] * ux[idx] + uy[idx] * uy[idx] + uz[idx] * uz[idx];
+                double usq = (double)ux[idx] * ux[idx] + (double)uy[idx] * uy[idx] +
+                             (double)uz[idx] * uz[idx];
                 if (usq > umax) umax = usq;
-                double p[19];
+                real p[19];
                 gather19(nx, ny, nz, phi, x, y, z, p);
                 /* src/logsave.c:135-137 */
                 double pres = pressure_at(c, p, rho[idx]);
-                if (phi[idx] >= 0.0) {
+                if (phi[idx] >= RL(0.0)) {
                     vol += 1.0;
                     sx += ux[idx];
                     sy += uy[idx];

@@ -6,6 +6,11 @@
  * path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
  * may load it; the product (libzsc.so) never links or calls it.
  *
+ * Built with -DZO_REAL=float (liboracle_f32.so), the step functions take
+ * float arrays and evaluate the same expressions in FP32, the precision of
+ * the reference's own kernels: the yardstick for what FP32 arithmetic of
+ * the reference's algorithm loses against FP64.
+ *
  * Layout: a fully periodic nx*ny*nz box without ghost planes, cell index
  * idx = x + nx*(y + ny*z) (x fastest, as src/gridid.c:32-35), populations
  * stored pop-major: pop i of cell idx lives at [i*N + idx], N = nx*ny*nz.
@@ -19,6 +24,13 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Element type of the lattices and moment fields: double, or float for the
+ * FP32 build (-DZO_REAL=float, liboracle_f32.so). */
+#ifndef ZO_REAL
+#define ZO_REAL double
+#endif
+typedef ZO_REAL zo_real;
 
 /* Scalars used by the step; names follow src/constants.h:62-85. */
 typedef struct zo_coef {
@@ -63,35 +75,35 @@ void zo_derive(double rho_l, double rho_h, double tau_rho, double tau_phi,
 void zo_init_phi(int nx, int ny, int nz, const double *bubbles, int nb,
                  double phi_star, double width, double *phi);
 
-void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-               double *f);
-void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-               const double *rho, double *g);
+void zo_init_f(int nx, int ny, int nz, const zo_coef *c, const zo_real *phi,
+               zo_real *f);
+void zo_init_g(int nx, int ny, int nz, const zo_coef *c, const zo_real *phi,
+               const zo_real *rho, zo_real *g);
 
-void zo_update_phi(int nx, int ny, int nz, const double *f, double *phi);
-void zo_update_rho(int nx, int ny, int nz, const double *g, double *rho);
-void zo_update_velocity(int nx, int ny, int nz, const double *g,
-                        const double *rho, double *ux, double *uy, double *uz);
+void zo_update_phi(int nx, int ny, int nz, const zo_real *f, zo_real *phi);
+void zo_update_rho(int nx, int ny, int nz, const zo_real *g, zo_real *rho);
+void zo_update_velocity(int nx, int ny, int nz, const zo_real *g,
+                        const zo_real *rho, zo_real *ux, zo_real *uy, zo_real *uz);
 
 /* A -> B collisions; grav_on selects grav (evolution) or 0 (relaxation). */
 void zo_collision_f(int nx, int ny, int nz, const zo_coef *c, int grav_on,
-                    const double *phi, const double *rho, const double *ux,
-                    const double *uy, const double *uz, const double *fa,
-                    double *fb);
+                    const zo_real *phi, const zo_real *rho, const zo_real *ux,
+                    const zo_real *uy, const zo_real *uz, const zo_real *fa,
+                    zo_real *fb);
 void zo_collision_g(int nx, int ny, int nz, const zo_coef *c, int grav_on,
-                    const double *phi, const double *rho, const double *ux,
-                    const double *uy, const double *uz, const double *ga,
-                    double *gb);
+                    const zo_real *phi, const zo_real *rho, const zo_real *ux,
+                    const zo_real *uy, const zo_real *uz, const zo_real *ga,
+                    zo_real *gb);
 
 /* B -> A streaming with periodic wrap. */
-void zo_stream_f(int nx, int ny, int nz, const zo_coef *c, const double *fb,
-                 double *fa);
-void zo_stream_g(int nx, int ny, int nz, const double *gb, double *ga);
+void zo_stream_f(int nx, int ny, int nz, const zo_coef *c, const zo_real *fb,
+                 zo_real *fa);
+void zo_stream_g(int nx, int ny, int nz, const zo_real *gb, zo_real *ga);
 
 /* nsteps full steps in the reference order (src/main.cu:496-590).  f, g hold
- * the A lattice on entry and exit; work must hold 7N + 19N + 5N doubles. */
+ * the A lattice on entry and exit; work must hold 7N + 19N + 5N elements. */
 void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
-            double *f, double *g, double *work);
+            zo_real *f, zo_real *g, zo_real *work);
 
 /* Diagnostics (src/stats.c:39-55, src/logsave.c:63-147) over moments.
  * out[0]  sum phi (all cells)        out[1]  sum rho (all cells)
@@ -101,12 +113,12 @@ void zo_run(int nx, int ny, int nz, const zo_coef *c, int grav_on, int nsteps,
  * out[7]  pressure sum inside (phi>=0)   out[8] pressure sum outside
  * out[9]  phi>=0 count over the full box */
 /* Pressure field of src/vtksave.c:75-119 from phi and rho (periodic). */
-void zo_pressure(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-                 const double *rho, double *p_out);
+void zo_pressure(int nx, int ny, int nz, const zo_coef *c, const zo_real *phi,
+                 const zo_real *rho, zo_real *p_out);
 
-void zo_stats(int nx, int ny, int nz, const zo_coef *c, const double *phi,
-              const double *rho, const double *ux, const double *uy,
-              const double *uz, double *out);
+void zo_stats(int nx, int ny, int nz, const zo_coef *c, const zo_real *phi,
+              const zo_real *rho, const zo_real *ux, const zo_real *uy,
+              const zo_real *uz, double *out);
 
 #ifdef __cplusplus
 }

@@ -181,6 +181,43 @@ def test_reference_input_files_reduced_precision(need_gpu, prec, tol):
     assert not bad, f"{prec}: {bad} above {tol}"
 
 
+def _errs(m, ref, p):
+    (phi, rho, *u), (phi_r, rho_r, *u_r) = m, ref
+    return {"phi": rel(phi, phi_r), "rho": rel(rho - p.rho_mean, rho_r - p.rho_mean),
+            "u": rel(np.stack(u), np.stack(u_r))}
+
+
+def test_reference_input_fp32_against_the_reference_fp32_arithmetic(need_gpu):
+    """The reference computes in float.  On its own input files (density
+    ratio 1000), 60 relaxation and 60 gravity steps, the errors against FP64
+    of: the reference's algorithm evaluated in FP32 (the oracle's FP32 build,
+    on the host), the GPU's FP32 step, and the GPU's f32d step (g stored as
+    deviations).  The GPU's FP32 step loses no more than the reference's own
+    FP32 arithmetic does (bound: 2x per field), and f32d keeps what both
+    lose (u at least 100x and rho - rho_mean 10x closer to FP64 than the
+    reference's FP32 arithmetic).  Measured (phi, rho - rho_mean, u):
+    reference FP32 arithmetic 2.9e-5, 9.3e-2, 2.7e-2; GPU f32 2.9e-5, 1.0e-1,
+    2.0e-2; GPU f32d 9.0e-6, 2.5e-3, 3.5e-6."""
+    dims, p, phi0 = reference_input_case()
+    s32 = O.State(phi0.shape, oracle_coef(p), dtype=np.float32).init(phi0)
+    s32.run(60, grav_on=False)
+    s32.run(60, grav_on=True)
+    ref32 = [np.asarray(m, dtype=np.float64) for m in s32.moments()]
+    out = {}
+    for q in ("f64", "f32", "f32d"):
+        with zsc.Context(*dims, precision=q) as ctx:
+            relax_then_rise(ctx, p, phi0, 60, 60)
+            out[q] = ctx.moments()
+    e_ref = _errs(ref32, out["f64"], p)
+    e32 = _errs(out["f32"], out["f64"], p)
+    ed = _errs(out["f32d"], out["f64"], p)
+    for label, e in (("reference FP32 arithmetic", e_ref), ("GPU f32", e32), ("GPU f32d", ed)):
+        print(label, {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    for k in e_ref:
+        assert e32[k] <= 2 * e_ref[k], k
+    assert ed["u"] <= e_ref["u"] / 100 and ed["rho"] <= e_ref["rho"] / 10
+
+
 def _dropin_tensors(n, dtype):
     import torch
     dev = "cuda"
