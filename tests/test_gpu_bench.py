@@ -40,7 +40,7 @@ def test_bench_json_line(need_gpu):
     for k in ("linear", "march"):
         assert rf[f"copy_{k}_GBs"] > 0
         assert abs(rf[f"frac_of_copy_{k}"] - rf["achieved"] / rf[f"copy_{k}_GBs"]) < 1e-3
-    assert cb["cpu_model"]
+    assert cb["cpu_model"] and cb["fp32"]["value"] > 0
     v = d["variants"]["bf16"]
     assert v["mlups"] > 0 and 0 < v["roofline"]["frac"] < 1
     assert "fp16" in v["storage"] and "bf16" in v["storage"]
