@@ -189,6 +189,10 @@ def test_driver_reports_missing_inputs(tmp_path):
     assert r.returncode == 1 and "cannot open" in r.stderr
     r = subprocess.run([exe, "-p", "f8"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
+    for prec in ("f64", "f32", "f32d", "bf16", "fp16"):  # accepted: fails on the inputs
+        r = subprocess.run([exe, "-p", prec, "-d", str(tmp_path)], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 1 and "cannot open" in r.stderr, prec
 
 
 def test_malformed_input_files_give_status_codes(tmp_path):
