@@ -183,12 +183,15 @@ def accuracy(w, params, steps, precs, ty, zchunk, flags):
 def adjoint_timing(steps: int, fwd_mlups_fp32: float):
     """MLUPS of the adjoint on the 128^3, 8-bubble workload (FP32): cells *
     steps / time of the backward pass with checkpoints every 20 steps (the
-    workload as specified; the recomputation of every segment included),
-    and of the adjoint steps alone (HIP events around each segment's
-    recomputation and its backward steps), after one untimed backward pass;
-    beside it (`tape`) the same with every step's state kept in HBM
-    (nothing to recompute; 90 GB at 400 steps).  Both modes give the same
-    gradients."""
+    workload as specified; the recomputation of every segment included,
+    running on a second hardware queue beside the previous segment's
+    adjoint steps), and of the adjoint steps alone (HIP events around each
+    segment's backward steps; with the overlap these intervals and those of
+    the recomputation run at the same time), after one untimed backward
+    pass; beside it the same with the recomputation in line
+    (`checkpoints_serial`, where the two kinds of interval add up to the
+    pass) and with every step's state kept in HBM (`tape`: nothing to
+    recompute; 90 GB at 400 steps).  All modes give the same gradients."""
     import torch
     from lbm_bench_gpu_amd import adjoint, configs, zsc
     w = configs.workload(4)
@@ -208,6 +211,7 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
            "fused_forward_mlups": round(fused, 1)}
     grads = {}
     modes = (("tape", {"tape": "auto"}), ("checkpoints", {"tape": False}),
+             ("checkpoints_serial", {"tape": False, "overlap": False}),
              # FP32 adjoint lattices over FP64 states: the FP64 forward's
              # gradients (1e-7 of central differences) at FP32 adjoint storage
              ("checkpoints_f32_over_f64_states", {"tape": False, "states": "f64"}),
@@ -238,20 +242,24 @@ def adjoint_timing(steps: int, fwd_mlups_fp32: float):
              "adjoint_step_mlups": round(adj_steps, 1),
              "adjoint_step_vs_forward": round(adj_steps / fused, 3),
              "recompute_ms": round(tm["recompute_ms"], 2),
-             "adjoint_steps_ms": round(tm["adjoint_steps_ms"], 2)}
+             "adjoint_steps_ms": round(tm["adjoint_steps_ms"], 2),
+             "recompute_overlapped": tm["overlapped"]}
         if mode == "checkpoints":
             # the workload as specified: checkpoints every 20 steps, recomputation
             out.update({"loss": L, "grad": dict(zip(adjoint.THETA, grads[mode])),
-                        "adjoint_step_mlups": r["adjoint_step_mlups"],
-                        "adjoint_step_vs_forward": r["adjoint_step_vs_forward"],
                         "adjoint_mlups": r["adjoint_mlups"],
                         "adjoint_vs_forward": r["adjoint_vs_forward"]})
+        if mode == "checkpoints_serial":
+            # the adjoint steps with nothing beside them
+            out.update({"adjoint_step_mlups": r["adjoint_step_mlups"],
+                        "adjoint_step_vs_forward": r["adjoint_step_vs_forward"]})
         if "states" in kw:
             r["grad"] = dict(zip(adjoint.THETA, grads[mode]))
         out[mode] = r
+        run.close()
         del run, a, lam
         torch.cuda.empty_cache()
-    out["grads_equal"] = grads["tape"] == grads["checkpoints"]
+    out["grads_equal"] = grads["tape"] == grads["checkpoints"] == grads["checkpoints_serial"]
     return out
 
 

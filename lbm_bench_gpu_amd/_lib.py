@@ -106,6 +106,7 @@ SIGNATURES = {
     "zsc_adj_forward": (_I, [_VP, _VP, _VP, _VP]),
     "zsc_adj_seed": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "zsc_adj_backward": (_I, [_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "zsc_adj_backward_steps": (_I, [_VP, _I, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "zsc_adj_flush": (_I, [_VP, _VP, _VP]),
     "zsc_adj_collide": (_I, [_VP, _VP, _VP, _VP]),
     "zsc_adj_fused": (_I, [_VP, _VP, _VP, _VP]),

@@ -75,12 +75,47 @@ def _vp(t: torch.Tensor):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _own_queue_stream(device):
+    """A stream on a hardware queue of its own, so that its kernels can run
+    beside those of the runner's stream.  Streams share the process's few
+    hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise), and two
+    streams that share one run one after the other; a stream created with a
+    compute-unit mask gets its own queue.  The mask names every compute unit,
+    so the stream may use the whole GPU; its priority is the default.  Falls
+    back to a pooled torch stream if the call is refused."""
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        ncu = torch.cuda.get_device_properties(device).multi_processor_count
+        words = (ncu + 31) // 32
+        mask = (ctypes.c_uint32 * words)(*([0xFFFFFFFF] * words))
+        if ncu % 32:
+            mask[words - 1] = (1 << (ncu % 32)) - 1
+        s = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            r = hip.hipExtStreamCreateWithCUMask(ctypes.byref(s), ctypes.c_uint32(words), mask)
+        if r != 0 or not s.value:
+            raise OSError(r)
+    except (OSError, AttributeError):
+        return torch.cuda.Stream(device=device)
+    st = torch.cuda.ExternalStream(s.value, device=device)
+    st._zsc_hip = hip  # the creator's handle, for the destroy below
+    return st
+
+
+def _destroy_stream(st):
+    hip = getattr(st, "_zsc_hip", None)
+    if hip is not None:
+        hip.hipStreamSynchronize(ctypes.c_void_p(st.cuda_stream))
+        hip.hipStreamDestroy(ctypes.c_void_p(st.cuda_stream))
+
+
 class AdjointRunner:
     """Owns a context and the device buffers of one differentiable run."""
 
     def __init__(self, dims, base: zsc.Params, precision: str = "f32", checkpoint_every: int = 20,
                  gravity_on: bool = True, device: str = "cuda", timing: bool = False,
-                 tape="auto", tape_fraction: float = 0.5, states: str | None = None):
+                 tape="auto", tape_fraction: float = 0.5, states: str | None = None,
+                 overlap: bool = True):
         """precision: storage of the adjoint lattices; states: precision of
         the forward run and its stored states (default: the same).  FP32
         adjoints over FP64 states (precision="f32", states="f64") keep the
@@ -93,7 +128,9 @@ class AdjointRunner:
         recomputation), False a checkpoint every ``checkpoint_every`` steps,
         "auto" the tape when it takes at most ``tape_fraction`` of the device
         memory that is free (to the driver or in torch's cache) when the
-        forward pass starts."""
+        forward pass starts.  overlap: with checkpoints, recompute the next
+        segment (on a second stream, into a second set of buffers) while
+        the adjoint steps of this one run."""
         states = states or precision
         if (precision, states) not in (("f32", "f32"), ("f64", "f64"), ("f32", "f64"),
                                        ("f32", "f32d")):
@@ -120,6 +157,10 @@ class AdjointRunner:
         self.ctx = zsc.Context(nx, ny, nz, precision=states, stream=self.stream.cuda_stream)
         check(lib().zsc_adj_set_lambda_precision(self.ctx._h, zsc._lib.PRECISIONS[precision]))
         self.ctx.set_gravity(gravity_on)
+        self.states_precision = states
+        self.gravity_on = bool(gravity_on)
+        self.overlap = bool(overlap)
+        self.stream2 = self.ctx2 = None  # the recomputation's, made on first use
         nbytes = lib().zsc_adj_scratch_bytes(self.ctx._h)
         self.scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
         self.grads = torch.zeros(16, dtype=torch.float64, device=device)
@@ -131,6 +172,22 @@ class AdjointRunner:
         self.timing = timing
         self.last_timing = None
 
+    def close(self):
+        """Frees the recomputation's context and stream (the runner's own
+        context frees itself)."""
+        if self.ctx2 is not None:
+            self.ctx2.close()
+            self.ctx2 = None
+        if self.stream2 is not None:
+            _destroy_stream(self.stream2)
+            self.stream2 = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def _lattice(self):
         return torch.empty((26, self.n), dtype=self.dtype, device=self.device)
 
@@ -140,6 +197,19 @@ class AdjointRunner:
     def _set_theta(self, theta):
         self.theta = [float(t) for t in theta]
         self.ctx.set_params(coefficients(self.theta, self.base))
+        if self.ctx2 is not None:
+            self.ctx2.set_params(coefficients(self.theta, self.base))
+
+    def _recompute_ctx(self):
+        """A second context on a second stream (default priority) for the
+        recomputation, with this run's parameters and gravity."""
+        if self.ctx2 is None:
+            self.stream2 = _own_queue_stream(self.device)
+            self.ctx2 = zsc.Context(*self.dims, precision=self.states_precision,
+                                    stream=self.stream2.cuda_stream)
+            self.ctx2.set_gravity(self.gravity_on)
+            self.ctx2.set_params(coefficients(self.theta, self.base))
+        return self.ctx2
 
     def forward(self, theta, phi0: torch.Tensor, nsteps: int, keep=True) -> torch.Tensor:
         """a_N; keeps checkpoints every k steps when ``keep``."""
@@ -232,13 +302,18 @@ class AdjointRunner:
         self.stream.synchronize()
         return g
 
-    def _recompute(self, s0, s1):
+    def _recompute(self, s0, s1, h=None, bank=None):
         """The input state and phi_n of every step of the segment s0 .. s1-1,
-        recomputed from its checkpoint."""
-        h = self.ctx._h
+        recomputed from its checkpoint (through context handle h, default
+        the runner's) into bank = (lattices, phis) or new buffers."""
+        h = self.ctx._h if h is None else h
+        out = iter(bank[0]) if bank is not None else None
+        outp = iter(bank[1]) if bank is not None else None
+        lat = (lambda: next(out)) if bank is not None else self._lattice
+        ph = (lambda: next(outp)) if bank is not None else self._phi
         states, phis = [], []
         if s0 == 0:
-            b, phi = self._lattice(), self._phi()
+            b, phi = lat(), ph()
             check(lib().zsc_adj_collide(h, _vp(self.checkpoints[0]), _vp(b), _vp(phi)))
             states.append(self.checkpoints[0])
             phis.append(phi)
@@ -247,12 +322,21 @@ class AdjointRunner:
             b = self.checkpoints[s0]
             n0 = s0
         for _ in range(n0, s1):
-            nb, phi = self._lattice(), self._phi()
+            nb, phi = lat(), ph()
             check(lib().zsc_adj_fused(h, _vp(b), _vp(nb), _vp(phi)))
             states.append(b)
             phis.append(phi)
             b = nb
         return states, phis
+
+    def _adjoint_steps(self, s0, states, phis, lam, lam_out, lam_tmp):
+        """lam_out = the adjoint at step s0 from lam at the segment's end:
+        one zsc_adj_backward_steps call (coefficient sums kept in scratch)."""
+        k = len(states)
+        sp = (ctypes.c_void_p * k)(*[t.data_ptr() for t in states])
+        pp = (ctypes.c_void_p * k)(*[t.data_ptr() if t is not None else None for t in phis])
+        check(lib().zsc_adj_backward_steps(self.ctx._h, k, sp, pp, 0 if s0 == 0 else 1, _vp(lam),
+                                           _vp(lam_out), _vp(lam_tmp), _vp(self.scratch), None))
 
     def _backward(self, lam, scale):
         if self.tape is None and 0 not in self.checkpoints:
@@ -262,42 +346,81 @@ class AdjointRunner:
         gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
         loss_slot = float(self.grads[14].item())
         self.grads.zero_()
-        lam2 = self._lam()
+        lam2, lam3 = self._lam(), self._lam()
         if self.tape is not None:
             # every step's input is on the tape: one segment, nothing to recompute
-            segments = [(0, self.nsteps, self.tape)]
+            segs = [(0, self.nsteps, self.tape)]
         else:
             starts = sorted(self.checkpoints)
-            segments = [(s0, starts[i + 1] if i + 1 < len(starts) else self.nsteps, None)
-                        for i, s0 in enumerate(starts)]
+            segs = [(s0, starts[i + 1] if i + 1 < len(starts) else self.nsteps, None)
+                    for i, s0 in enumerate(starts)]
+        segs = segs[::-1]
+        overlap = self.tape is None and self.overlap and len(segs) > 1
+        ev = lambda: torch.cuda.Event(enable_timing=self.timing)
         marks = []
-        for s0, s1, taped in reversed(segments):
-            if self.timing:
-                ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-                ev[0].record(self.stream)
-                marks.append(ev)
-            if taped is not None:
-                states, phis = taped
-            else:
-                states, phis = self._recompute(s0, s1)
-            if self.timing:
-                marks[-1][1].record(self.stream)
-            for n in range(s1 - 1, s0 - 1, -1):
-                j = n - s0
-                check(lib().zsc_adj_backward(h, _vp(states[j]), 0 if n == 0 else 1,
-                                             _vp(phis[j]), _vp(lam), _vp(lam2),
-                                             _vp(self.scratch), None))
+        if overlap:
+            # segment i is recomputed on stream2 into bank i % 2 while segment
+            # i - 1's adjoint steps run on the runner's stream; bank i % 2 is
+            # rewritten only after segment i - 2's adjoint steps are done
+            h2 = self._recompute_ctx()._h
+            k = max(s1 - s0 for s0, s1, _ in segs)
+            banks = [([self._lattice() for _ in range(k)], [self._phi() for _ in range(k)])
+                     for _ in range(2)]
+            done_rec, done_adj, t_rec = {}, {}, []
+
+            def issue(i):
+                s0, s1, _ = segs[i]
+                with torch.cuda.stream(self.stream2):
+                    if i >= 2:
+                        self.stream2.wait_event(done_adj[i - 2])
+                    t0 = ev()
+                    t0.record(self.stream2)
+                    st = self._recompute(s0, s1, h2, banks[i % 2])
+                    done_rec[i] = ev()
+                    done_rec[i].record(self.stream2)
+                    t_rec.append((t0, done_rec[i]))
+                return st
+
+            pending = issue(0)
+            for i, (s0, s1, _) in enumerate(segs):
+                states, phis = pending
+                if i + 1 < len(segs):
+                    pending = issue(i + 1)
+                self.stream.wait_event(done_rec[i])
+                t0 = ev()
+                t0.record(self.stream)
+                self._adjoint_steps(s0, states, phis, lam, lam2, lam3)
                 lam, lam2 = lam2, lam
-            if self.timing:
-                marks[-1][2].record(self.stream)
-            del states, phis
+                done_adj[i] = ev()
+                done_adj[i].record(self.stream)
+                marks.append((t0, done_adj[i]))
+        else:
+            t_rec = []
+            for s0, s1, taped in segs:
+                t0 = ev()
+                t0.record(self.stream)
+                if taped is not None:
+                    states, phis = taped
+                else:
+                    states, phis = self._recompute(s0, s1)
+                t1 = ev()
+                t1.record(self.stream)
+                self._adjoint_steps(s0, states, phis, lam, lam2, lam3)
+                lam, lam2 = lam2, lam
+                t2 = ev()
+                t2.record(self.stream)
+                t_rec.append((t0, t1))
+                marks.append((t1, t2))
+                del states, phis
         check(lib().zsc_adj_flush(h, _vp(self.scratch), gp))
         check(lib().zsc_adj_init_backward(h, _vp(self.phi0), _vp(lam), _vp(self.scratch), gp))
         if self.timing:
             self.stream.synchronize()
+            # with overlap the two kinds of interval run at the same time
             self.last_timing = {
-                "recompute_ms": sum(m[0].elapsed_time(m[1]) for m in marks),
-                "adjoint_steps_ms": sum(m[1].elapsed_time(m[2]) for m in marks)}
+                "recompute_ms": sum(a.elapsed_time(b) for a, b in t_rec),
+                "adjoint_steps_ms": sum(a.elapsed_time(b) for a, b in marks),
+                "overlapped": overlap}
         g = self.grads.cpu().numpy()
         g[14] = loss_slot
         self.lam0 = lam

@@ -273,10 +273,12 @@ __device__ inline void block_reduce_grads(double *ap, double *partial, bool accu
 // Blocks of the adjoint kernels: kAdjCellsPerThread cells per thread (the
 // per-block reduction of the coefficient sums, ~15% of pass 1's VALU work
 // for one cell, is paid once per thread; 2 times 3.6% faster than 1 and
-// 1.5% faster than 4, whose last round of blocks runs part-empty), a
-// multiple of 8 blocks for the XCD-contiguous order, more cells per thread
-// beyond kAdjMaxBlocks blocks (per-thread sums at precision R, block and
-// global sums in FP64).  The partial buffers hold kAdjMaxBlocks rows.
+// 1.5% faster than 4, whose last round of blocks runs part-empty; a grid of
+// one round, the blocks the GPU holds at once with about 11 cells per
+// thread, is 2-4% slower), a multiple of 8 blocks for the XCD-contiguous
+// order, more cells per thread beyond kAdjMaxBlocks blocks (per-thread sums
+// at precision R, block and global sums in FP64).  The partial buffers hold
+// kAdjMaxBlocks rows.
 constexpr int kAdjMaxBlocks = 65536;
 constexpr int kAdjCellsPerThread = 2;
 inline int adj_blocks(int64_t n)
@@ -287,20 +289,28 @@ inline int adj_blocks(int64_t n)
     return (int)(b < 8 ? 8 : (b > kAdjMaxBlocks ? kAdjMaxBlocks : b));
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs.  With `remap` the
-// grid-stride loop gives each XCD a contiguous eighth of every sweep, so the
-// neighbouring rows and planes that a cell's pulls and stencils read were
-// mostly fetched into the same XCD's L2 (the per-block partial sums stay
-// indexed by blockIdx).
-__device__ inline int64_t adj_first(int remap)
+// Cells of a pass 1 thread.  Blocks are dispatched round-robin over the 8
+// XCDs; with `remap` the blocks of XCD j (blockIdx % 8 == j) sweep the j-th
+// contiguous eighth of the cells in order, so the rows and planes next to
+// the cells being updated were mostly fetched into that XCD's L2, except at
+// the eighth's two ends (the per-block partial sums stay indexed by
+// blockIdx).
+struct AdjRange {
+    int64_t first, end, stride;
+};
+__device__ inline AdjRange adj_range(int64_t n, int remap)
 {
-    int64_t b = blockIdx.x;
     const int64_t nb = gridDim.x;
-    if (remap && (nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-    return b * blockDim.x + threadIdx.x;
+    if (remap && (nb & 7) == 0) {
+        const int64_t j = blockIdx.x & 7, lb = blockIdx.x >> 3, per = (n + 7) >> 3;
+        const int64_t e = (j + 1) * per;
+        return {j * per + lb * blockDim.x + threadIdx.x, e < n ? e : n, (nb >> 3) * blockDim.x};
+    }
+    return {(int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, nb * blockDim.x};
 }
-#define ZSC_ADJ_STRIDE(idx, n, remap) \
-    for (int64_t idx = adj_first(remap); idx < (n); idx += (int64_t)gridDim.x * blockDim.x)
+#define ZSC_ADJ_RANGE(idx, n, remap)                                             \
+    for (AdjRange rg_ = adj_range((n), (remap)), *rp_ = &rg_; rp_; rp_ = nullptr) \
+        for (int64_t idx = rg_.first; idx < rg_.end; idx += rg_.stride)
 
 // Pass 1: lam_out (local part, including phi's local uses), the stencil-
 // adjoint fields a_lap, a_grad (4 fields) and per-block coefficient
@@ -309,23 +319,31 @@ __device__ inline int64_t adj_first(int remap)
 // stored at precision L (L = R, or FP32 adjoints over FP64 states).
 // FP64 states need more registers than the file holds at two waves per SIMD;
 // the bound keeps two resident (spilling what does not fit) rather than one.
+// FP32 states are held to three (168 registers): left free, the compiler
+// takes a few more and runs two, 16% slower per adjoint step at 128^3.
 template <class R>
 constexpr int adj_waves()
 {
-    return sizeof(R) == 8 ? 2 : 1;
+    return sizeof(R) == 8 ? 2 : 3;
 }
 
-template <class R, class L, bool PULL, bool DEV = false>
+//
+// adef (may be null): the next step's stencil term a = lap[a_lap] -
+// div[a_grad], not yet added to lam's seven f populations (pass 2 of that
+// step wrote it as one field instead); it is added here as each f value is
+// loaded, in the adjoint-lattice precision, which is the sum pass 2 would
+// have stored.
+template <class R, class L, bool PULL, bool DEV = false, bool DEFER = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves<R>())))
     k_adj_pass1(int nx, int ny, int nz, int64_t n, Coef<R> c, Coef<L> cl, int grav_on,
                 const R *__restrict__ A, const R *__restrict__ phi, const L *__restrict__ lam,
-                L *__restrict__ lam_out, L *__restrict__ fields, double *partial, int accumulate,
-                int remap)
+                const L *__restrict__ adef, L *__restrict__ lam_out, L *__restrict__ fields,
+                double *partial, int accumulate, int remap)
 {
     R ap[kNumGrad];
 #pragma unroll
     for (int j = 0; j < kNumGrad; ++j) ap[j] = R(0);
-    ZSC_ADJ_STRIDE(idx, n, remap)
+    ZSC_ADJ_RANGE(idx, n, remap)
     {
         int x, y, z;
         cell_xyz32((uint32_t)idx, nx, ny, x, y, z);
@@ -351,15 +369,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_wa
         }
         // S^T: gather the next adjoint from downwind
         R lfd[7];
-        lf[0] = ld32z<0>(lam, own);
         lfd[0] = R(0);
+        if constexpr (DEFER) {
+            const L a0 = ld32z<0>(adef, own);
+            lf[0] = ld32z<0>(lam, own) + a0;
 #pragma unroll
-        for (int i = 1; i < 7; ++i) {
-            const L *li = lam + (int64_t)i * n;
-            const L mine = ld32z<0>(li, own);
-            const L down = ld32z<0>(li, nb.at(cx(i), cy(i), cz(i)));
-            lf[i] = cl.eta * down + cl.eta2 * mine;
-            lfd[i] = (R)down - (R)mine;  // for the eta derivative below
+            for (int i = 1; i < 7; ++i) {
+                const L *li = lam + (int64_t)i * n;
+                const uint32_t od = nb.at(cx(i), cy(i), cz(i));
+                const L mine = ld32z<0>(li, own) + a0;
+                const L down = ld32z<0>(li, od) + ld32z<0>(adef, od);
+                lf[i] = cl.eta * down + cl.eta2 * mine;
+                lfd[i] = (R)down - (R)mine;
+            }
+        } else {
+            lf[0] = ld32z<0>(lam, own);
+#pragma unroll
+            for (int i = 1; i < 7; ++i) {
+                const L *li = lam + (int64_t)i * n;
+                const L mine = ld32z<0>(li, own);
+                const L down = ld32z<0>(li, nb.at(cx(i), cy(i), cz(i)));
+                lf[i] = cl.eta * down + cl.eta2 * mine;
+                lfd[i] = (R)down - (R)mine;  // for the eta derivative below
+            }
         }
 #pragma unroll
         for (int i = 0; i < 19; ++i)
@@ -382,16 +414,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_wa
     block_reduce_grads<256>(apd, partial, accumulate != 0);
 }
 
-// Pass 2: a_phi's stencil part lap[a_lap] - div[a_grad] (the transposed
-// stencils), added to the adjoint of every f population.  Tiles of 64 x TY
-// cells march along z; the four stencil fields of planes k-1, k, k+1 (tile
-// and 1-cell ring) sit in a 4-slot LDS ring, one barrier per plane, so each
-// field value is loaded from memory about once instead of once per stencil
-// point.  Row offsets are 32-bit (rows < 4 GiB, checked at launch).
-template <class R, int TY>
+// Pass 2: a_phi's stencil part a = lap[a_lap] - div[a_grad] (the
+// transposed stencils), added to the adjoint of every f population, or, with
+// DEFER, written as one field (aout) for the next backward step's pass 1 to
+// add as it loads those populations: 20 bytes per cell instead of 72.
+// Tiles of 64 x TY cells march along z; the four stencil fields of planes
+// k-1, k, k+1 (tile and 1-cell ring) sit in a 4-slot LDS ring, one barrier
+// per plane, so each field value is loaded from memory about once instead
+// of once per stencil point.  Row offsets are 32-bit (rows < 4 GiB, checked
+// at launch).
+template <class R, int TY, bool DEFER = false>
 __global__ void __launch_bounds__(64 * TY)
     k_adj_pass2(int nx, int ny, int nz, int64_t n, int zchunk, int nbx, int nby, int nblk,
-                int remap, const R *__restrict__ fields, R *__restrict__ lam_out)
+                int remap, const R *__restrict__ fields, R *__restrict__ lam_out,
+                R *__restrict__ aout)
 {
     __shared__ R sh[4][4][TY + 2][66];  // [slot][field][row][column]
     const int b = blockIdx.x;
@@ -454,7 +490,7 @@ __global__ void __launch_bounds__(64 * TY)
     for (int k = z0; k < z1; ++k) {
         const uint32_t o = (uint32_t)k * nxy + (uint32_t)y * (uint32_t)nx + (uint32_t)x;
         R lf[7];
-        if (own) {
+        if (own && !DEFER) {
 #pragma unroll
             for (int i = 0; i < 7; ++i) lf[i] = ld32z<0>(lam_out + (int64_t)i * n, o);
         }
@@ -478,8 +514,12 @@ __global__ void __launch_bounds__(64 * TY)
             gather(3, h);
             dv += R(2) * (h[5] - h[6]) + h[11] - h[12] + h[14] - h[13] + h[15] - h[16] + h[18] - h[17];
             a -= dv * W<R>::inv12;
+            if constexpr (DEFER) {
+                st32z<0>(aout, o, a);
+            } else {
 #pragma unroll
-            for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, o, lf[i] + a);
+                for (int i = 0; i < 7; ++i) st32z<0>(lam_out + (int64_t)i * n, o, lf[i] + a);
+            }
         }
     }
 }
@@ -575,8 +615,14 @@ __global__ void __launch_bounds__(256) k_adj_accumulate(int nblocks, const doubl
 // adjoint lattices and stencil fields (R, or float over double states).
 template <class R, class L>
 struct Adj {
-    // scratch: 4n L stencil fields | partials | deferred partials
-    static size_t field_bytes(int64_t n) { return align256((size_t)n * 4 * sizeof(L)); }
+    // scratch: 4n L stencil fields | n L deferred stencil term | partials |
+    // deferred partials.  A step's pass 1 reads the term of the step after
+    // it before its pass 2 writes its own there.
+    static size_t field_bytes(int64_t n) { return align256((size_t)n * 5 * sizeof(L)); }
+    static L *term(const Geo &g, void *scratch)
+    {
+        return (L *)scratch + (size_t)g.n * 4;
+    }
     static size_t scratch_bytes(int64_t n)
     {
         return field_bytes(n) + 2 * (size_t)kAdjMaxBlocks * kNumGrad * sizeof(double);
@@ -591,9 +637,13 @@ struct Adj {
     // grads == nullptr defers the coefficient sums into per-block partials
     // kept in the scratch buffer until flush(); defer_first: no deferred
     // sums exist yet, so the partials are written rather than added to.
+    // a_in (may be null): lam's deferred stencil term; a_out (may be null):
+    // where this step leaves its own stencil term instead of adding it to
+    // lam_out (then lam_out's f populations lack it).
     static hipError_t backward(const Geo &g, const zsc_params &p, int grav_on, hipStream_t s,
                                const R *A, bool pulled, const R *phi, const L *lam, L *lam_out,
-                               void *scratch, double *grads, bool defer_first, bool dev = false)
+                               void *scratch, double *grads, bool defer_first, bool dev = false,
+                               const L *a_in = nullptr, L *a_out = nullptr)
     {
         if ((uint64_t)g.n * sizeof(R) > ((uint64_t)1 << 32)) return hipErrorInvalidValue;
         L *fields = (L *)scratch;
@@ -605,12 +655,19 @@ struct Adj {
         const int remap = 1;
         auto pass1 = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, s, g.nx, g.ny, g.nz, g.n, c, cl,
-                               grav_on, A, phi, lam, lam_out, fields, partial, acc, remap);
+                               grav_on, A, phi, lam, a_in, lam_out, fields, partial, acc, remap);
+        };
+        auto pick = [&](auto dev_tag) {
+            constexpr bool D = decltype(dev_tag)::value;
+            if (a_in)
+                pulled ? pass1(k_adj_pass1<R, L, true, D, true>) : pass1(k_adj_pass1<R, L, false, D, true>);
+            else
+                pulled ? pass1(k_adj_pass1<R, L, true, D>) : pass1(k_adj_pass1<R, L, false, D>);
         };
         if (!dev)
-            pulled ? pass1(k_adj_pass1<R, L, true>) : pass1(k_adj_pass1<R, L, false>);
+            pick(std::false_type{});
         else if constexpr (sizeof(R) == 4)  // deviation states exist at FP32 only
-            pulled ? pass1(k_adj_pass1<R, L, true, true>) : pass1(k_adj_pass1<R, L, false, true>);
+            pick(std::true_type{});
         else
             return hipErrorInvalidValue;
         {
@@ -621,9 +678,14 @@ struct Adj {
                 zchunk /= 2;
             const int64_t nblk = (int64_t)nbx * nby * ((g.nz + zchunk - 1) / zchunk);
             if (nblk > 0x7fffffff) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_adj_pass2<L, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
-                               g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk, remap,
-                               (const L *)fields, lam_out);
+            if (a_out)
+                hipLaunchKernelGGL((k_adj_pass2<L, TY, true>), dim3((unsigned)nblk), dim3(64, TY), 0,
+                                   s, g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk, remap,
+                                   (const L *)fields, lam_out, a_out);
+            else
+                hipLaunchKernelGGL((k_adj_pass2<L, TY>), dim3((unsigned)nblk), dim3(64, TY), 0, s,
+                                   g.nx, g.ny, g.nz, g.n, zchunk, nbx, nby, (int)nblk, remap,
+                                   (const L *)fields, lam_out, (L *)nullptr);
         }
         if (grads)
             hipLaunchKernelGGL(k_adj_accumulate<R>, dim3(1), dim3(256), 0, s, nb, partial, grads);

@@ -1434,6 +1434,47 @@ extern "C" int zsc_adj_backward(zsc_ctx *c, const void *state, int pulled, const
     });
 }
 
+extern "C" int zsc_adj_backward_steps(zsc_ctx *c, int k, const void *const *states,
+                                      const void *const *phis, int first_pulled,
+                                      const void *lam_next, void *lam_out, void *lam_tmp,
+                                      void *scratch, double *grads)
+{
+    if (k < 1 || !states || !phis || !lam_next || !lam_out || !scratch || lam_next == lam_out ||
+        (k > 1 && (!lam_tmp || lam_tmp == lam_out || lam_tmp == lam_next)))
+        return fail(ZSC_ERR_ARG, "zsc_adj_backward_steps: bad step count, null or aliased pointer");
+    for (int j = 0; j < k; ++j)
+        if (!states[j] || ((j > 0 || first_pulled) && !phis[j]))
+            return fail(ZSC_ERR_ARG, "zsc_adj_backward_steps: null state or phi of step %d", j);
+    return adj_lam_call(c, "zsc_adj_backward_steps", [&](auto tag, auto ltag) {
+        using T = ZSC_T;
+        using L = typename decltype(ltag)::type;
+        void *adj_scr = (char *)scratch + align256((size_t)c->geo.n * sizeof(T));
+        L *term = Adj<T, L>::term(c->geo, adj_scr);
+        const L *lin = (const L *)lam_next;
+        const L *a_in = nullptr;
+        for (int j = k - 1; j >= 0; --j) {
+            // the step of j = 0 writes lam_out, the ones before alternate
+            L *lout = (L *)((j & 1) == 0 ? lam_out : lam_tmp);
+            const bool pulled = j > 0 || first_pulled;
+            const void *phi_n = phis[j];
+            if (!phi_n) {
+                hipError_t e = Ops<T>::lat_phi(c->geo, c->stream, states[j], scratch);
+                if (e != hipSuccess) return e;
+                phi_n = scratch;
+            }
+            hipError_t e = Adj<T, L>::backward(c->geo, c->params, c->grav_on, c->stream,
+                                               (const T *)states[j], pulled, (const T *)phi_n, lin,
+                                               lout, adj_scr, grads, !grads && !c->adj_deferred,
+                                               adj_dev(c), a_in, j > 0 ? term : nullptr);
+            if (e != hipSuccess) return e;
+            if (!grads) c->adj_deferred = true;
+            lin = lout;
+            a_in = term;
+        }
+        return hipSuccess;
+    });
+}
+
 extern "C" int zsc_adj_flush(zsc_ctx *c, void *scratch, double *grads)
 {
     if (!scratch || !grads) return fail(ZSC_ERR_ARG, "zsc_adj_flush: null pointer");

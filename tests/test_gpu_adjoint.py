@@ -111,17 +111,97 @@ def test_tape_equals_recomputation(need_gpu, precision):
     theta = adjoint.theta_of(p)
     tt = [theta[0] * 1.1, theta[1] * 0.95, theta[2], theta[3], theta[4] * 1.3]
     out = {}
-    for tape in (True, False):
-        run = adjoint.AdjointRunner(dims, p, checkpoint_every=4, tape=tape, **_runner_kw(precision))
+    # the checkpointed pass with its recomputation on a second stream
+    # (overlapping the adjoint steps) and without
+    for mode in ("tape", "checkpoints", "checkpoints-overlap"):
+        tape = mode == "tape"
+        run = adjoint.AdjointRunner(dims, p, checkpoint_every=4, tape=tape,
+                                    overlap=mode.endswith("overlap"), **_runner_kw(precision))
         tgt = run.forward(tt, torch.from_numpy(phi0), 11, keep=False)[:7].sum(0).clone()
         L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, 11)
         assert (run.tape is not None) == tape
         assert run.tape is not None or sorted(run.checkpoints) == [0, 4, 8]
-        out[tape] = (L, grads, run.lam0.cpu())
-    print(precision, "tape", out[True][1], "\ncheckpoints", out[False][1], flush=True)
-    assert out[True][0] == out[False][0]
-    assert out[True][1] == out[False][1]
-    assert torch.equal(out[True][2], out[False][2])
+        out[mode] = (L, grads, run.lam0.cpu())
+    print(precision, {m: v[1] for m, v in out.items()}, flush=True)
+    for mode in ("checkpoints", "checkpoints-overlap"):
+        assert out["tape"][0] == out[mode][0]
+        assert out["tape"][1] == out[mode][1]
+        assert torch.equal(out["tape"][2], out[mode][2])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32/f64", "f32/f32d"])
+def test_backward_steps_equal_single_steps(need_gpu, precision):
+    """zsc_adj_backward_steps (each step's stencil term left as one field
+    for the next step to add) equals one zsc_adj_backward call per step bit
+    for bit, in dL/da and in the coefficient sums: a whole 7-step run from
+    a_0 (unpulled first state), a 3-step range of pulled states, and a
+    single step whose phi the call computes."""
+    import torch
+    dims = (14, 10, 8)
+    p, phi0 = small_problem(dims, seed=7)
+    run = adjoint.AdjointRunner(dims, p, checkpoint_every=4, tape=True, **_runner_kw(precision))
+    run.forward(adjoint.theta_of(p), torch.from_numpy(phi0), 7)
+    states, phis = run.tape
+    h = run.ctx._h
+    g = torch.Generator(device="cuda").manual_seed(11)
+    lam_n = torch.randn((26, run.n), dtype=run.ldtype, device="cuda", generator=g)
+    gp = ctypes.cast(run.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
+    cases = [(0, 7, True), (2, 5, True), (0, 1, False)]
+    for n0, n1, with_phi in cases:
+        out = {}
+        for mode in ("single", "steps"):
+            run.grads.zero_()
+            lam, lam2, lam3 = lam_n.clone(), run._lam(), run._lam()
+            torch.cuda.synchronize()
+            if mode == "single":
+                for n in range(n1 - 1, n0 - 1, -1):
+                    phi = _vp(phis[n]) if (with_phi or n > 0) else None
+                    check(lib().zsc_adj_backward(h, _vp(states[n]), 0 if n == 0 else 1, phi,
+                                                 _vp(lam), _vp(lam2), _vp(run.scratch), gp))
+                    lam, lam2 = lam2, lam
+                res = lam
+            else:
+                k = n1 - n0
+                sp = (ctypes.c_void_p * k)(*[states[n].data_ptr() for n in range(n0, n1)])
+                pp = (ctypes.c_void_p * k)(*[phis[n].data_ptr() if (with_phi or n > 0) else None
+                                             for n in range(n0, n1)])
+                check(lib().zsc_adj_backward_steps(h, k, sp, pp, 0 if n0 == 0 else 1, _vp(lam),
+                                                   _vp(lam2), _vp(lam3), _vp(run.scratch), gp))
+                res = lam2
+                run.ctx.sync()
+                assert torch.equal(lam, lam_n)  # lam_next is not modified
+            run.ctx.sync()
+            out[mode] = (res.clone(), run.grads.clone())
+        d = (out["single"][0] - out["steps"][0]).abs().max().item()
+        print(precision, (n0, n1), "max |d lam|", d, flush=True)
+        assert torch.equal(out["single"][0], out["steps"][0])
+        assert torch.equal(out["single"][1], out["steps"][1])
+
+
+def test_backward_steps_arguments(need_gpu):
+    """Bad step counts, missing phi of a pulled state and aliased adjoint
+    lattices are refused with ZSC_ERR_ARG before anything runs."""
+    import torch
+    from lbm_bench_gpu_amd._lib import ZSC_ERR_ARG, ZscError
+    dims = (12, 10, 8)
+    p, phi0 = small_problem(dims, seed=8)
+    run = adjoint.AdjointRunner(dims, p, precision="f32", checkpoint_every=4, tape=True)
+    run.forward(adjoint.theta_of(p), torch.from_numpy(phi0), 3)
+    states, phis = run.tape
+    h = run.ctx._h
+    lam, lam2, lam3 = run._lam(), run._lam(), run._lam()
+    sp = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in states])
+    pp = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in phis])
+    nophi = (ctypes.c_void_p * 3)(phis[0].data_ptr(), None, phis[2].data_ptr())
+    bad = [(0, sp, pp, 0, lam, lam2, lam3), (3, sp, nophi, 0, lam, lam2, lam3),
+           (3, sp, pp, 0, lam, lam, lam3), (3, sp, pp, 0, lam, lam2, None),
+           (3, sp, pp, 0, lam, lam2, lam2), (3, sp, pp, 0, lam, lam2, lam)]
+    for k, s_, p_, fp, a, b, c in bad:
+        with pytest.raises(ZscError) as e:
+            check(lib().zsc_adj_backward_steps(h, k, s_, p_, fp, _vp(a), _vp(b),
+                                               None if c is None else _vp(c), _vp(run.scratch),
+                                               None))
+        assert e.value.code == ZSC_ERR_ARG
 
 
 def _workload_fd(precision, gravity_on):
