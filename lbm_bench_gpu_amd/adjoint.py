@@ -125,7 +125,8 @@ class AdjointRunner:
         throughout and keep the density variations that FP32 states round
         away, with the coefficient sums formed without the rest state's
         part (it cancels in the tau_g derivative).  tape: True keeps every step's input state and phi (no
-        recomputation), False a checkpoint every ``checkpoint_every`` steps,
+        recomputation), False a checkpoint every ``checkpoint_every`` steps
+        and the states of the last segment (its backward steps come first),
         "auto" the tape when it takes at most ``tape_fraction`` of the device
         memory that is free (to the driver or in torch's cache) when the
         forward pass starts.  overlap: with checkpoints, recompute the next
@@ -242,7 +243,7 @@ class AdjointRunner:
         self._set_theta(theta)
         self.phi0 = phi0.to(self.device, self.dtype).contiguous().view(-1)
         h = self.ctx._h
-        self.checkpoints, self.tape = {}, None
+        self.checkpoints, self.tape, self.tail = {}, None, None
         a0 = self._lattice()
         check(lib().zsc_adj_init(h, _vp(self.phi0), _vp(a0)))
         self.checkpoints = {0: a0} if keep else {}
@@ -266,15 +267,30 @@ class AdjointRunner:
             self.tape = (states, phis)
             self.final = a
             return a
+        # checkpoints; the last segment's states and phi are kept as they are
+        # computed (its backward steps come first, and need no recomputation)
+        last = (nsteps - 1) // self.k * self.k
         b, b2, phi = self._lattice(), self._lattice(), self._phi()
         check(lib().zsc_adj_collide(h, _vp(a0), _vp(b), _vp(phi)))
+        tail = ([a0], [phi]) if keep and last == 0 else None
         for s in range(1, nsteps):
-            if keep and s % self.k == 0:
+            if keep and s == last:
+                self.checkpoints[s] = b  # kept in the tail, never written again
+                tail = ([], [])
+            elif keep and s % self.k == 0:
                 self.checkpoints[s] = b.clone()
-            check(lib().zsc_adj_fused(h, _vp(b), _vp(b2), None))
-            b, b2 = b2, b
+            if tail is not None:
+                nb, ph = self._lattice(), self._phi()
+                check(lib().zsc_adj_fused(h, _vp(b), _vp(nb), _vp(ph)))
+                tail[0].append(b)
+                tail[1].append(ph)
+                b = nb
+            else:
+                check(lib().zsc_adj_fused(h, _vp(b), _vp(b2), None))
+                b, b2 = b2, b
         a = b2
         check(lib().zsc_adj_stream(h, _vp(b), _vp(a)))
+        self.tail = tail
         self.final = a
         return a
 
@@ -354,6 +370,8 @@ class AdjointRunner:
             starts = sorted(self.checkpoints)
             segs = [(s0, starts[i + 1] if i + 1 < len(starts) else self.nsteps, None)
                     for i, s0 in enumerate(starts)]
+            if self.tail is not None:
+                segs[-1] = segs[-1][:2] + (self.tail,)
         segs = segs[::-1]
         overlap = self.tape is None and self.overlap and len(segs) > 1
         ev = lambda: torch.cuda.Event(enable_timing=self.timing)
@@ -363,13 +381,16 @@ class AdjointRunner:
             # i - 1's adjoint steps run on the runner's stream; bank i % 2 is
             # rewritten only after segment i - 2's adjoint steps are done
             h2 = self._recompute_ctx()._h
-            k = max(s1 - s0 for s0, s1, _ in segs)
+            k = max(s1 - s0 for s0, s1, taped in segs if taped is None)
             banks = [([self._lattice() for _ in range(k)], [self._phi() for _ in range(k)])
                      for _ in range(2)]
             done_rec, done_adj, t_rec = {}, {}, []
 
             def issue(i):
-                s0, s1, _ = segs[i]
+                s0, s1, taped = segs[i]
+                if taped is not None:  # the last segment, kept by the forward pass
+                    done_rec[i] = None
+                    return taped
                 with torch.cuda.stream(self.stream2):
                     if i >= 2:
                         self.stream2.wait_event(done_adj[i - 2])
@@ -386,7 +407,8 @@ class AdjointRunner:
                 states, phis = pending
                 if i + 1 < len(segs):
                     pending = issue(i + 1)
-                self.stream.wait_event(done_rec[i])
+                if done_rec[i] is not None:
+                    self.stream.wait_event(done_rec[i])
                 t0 = ev()
                 t0.record(self.stream)
                 self._adjoint_steps(s0, states, phis, lam, lam2, lam3)
