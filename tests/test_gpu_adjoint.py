@@ -104,7 +104,9 @@ def test_tape_equals_recomputation(need_gpu, precision):
     """The tape holds the same states and phi_n that the checkpointed
     backward pass recomputes (the same kernels on the same inputs), so both
     give the same gradients and the same dL/da_0 bit for bit; segments of 4
-    with a ragged last one (11 steps)."""
+    with a ragged last one (23 steps: the last segment kept from the forward
+    pass, five recomputed, so the overlapped pass reuses each of its two
+    buffer sets)."""
     import torch
     dims = (14, 10, 8)
     p, phi0 = small_problem(dims, seed=2)
@@ -117,10 +119,11 @@ def test_tape_equals_recomputation(need_gpu, precision):
         tape = mode == "tape"
         run = adjoint.AdjointRunner(dims, p, checkpoint_every=4, tape=tape,
                                     overlap=mode.endswith("overlap"), **_runner_kw(precision))
-        tgt = run.forward(tt, torch.from_numpy(phi0), 11, keep=False)[:7].sum(0).clone()
-        L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, 11)
+        tgt = run.forward(tt, torch.from_numpy(phi0), 23, keep=False)[:7].sum(0).clone()
+        L, grads = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, 23)
         assert (run.tape is not None) == tape
-        assert run.tape is not None or sorted(run.checkpoints) == [0, 4, 8]
+        assert run.tape is not None or (sorted(run.checkpoints) == [0, 4, 8, 12, 16, 20]
+                                        and len(run.tail[0]) == 3)
         out[mode] = (L, grads, run.lam0.cpu())
     print(precision, {m: v[1] for m, v in out.items()}, flush=True)
     for mode in ("checkpoints", "checkpoints-overlap"):
