@@ -131,7 +131,8 @@ class AdjointRunner:
         memory that is free (to the driver or in torch's cache) when the
         forward pass starts.  overlap: with checkpoints, recompute the next
         segment (on a second stream, into a second set of buffers) while
-        the adjoint steps of this one run."""
+        the adjoint steps of this one run, when both sets of buffers take at
+        most 80% of the free device memory (otherwise in line)."""
         states = states or precision
         if (precision, states) not in (("f32", "f32"), ("f64", "f64"), ("f32", "f64"),
                                        ("f32", "f32d")):
@@ -374,6 +375,13 @@ class AdjointRunner:
                 segs[-1] = segs[-1][:2] + (self.tail,)
         segs = segs[::-1]
         overlap = self.tape is None and self.overlap and len(segs) > 1
+        if overlap:
+            # two sets of segment buffers instead of one: only where they fit
+            esize = 4 if self.dtype == torch.float32 else 8
+            need = 2 * self.k * 27 * self.n * esize
+            free, _ = torch.cuda.mem_get_info(self.device)
+            cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+            overlap = need <= 0.8 * (free + cached)
         ev = lambda: torch.cuda.Event(enable_timing=self.timing)
         marks = []
         if overlap:
