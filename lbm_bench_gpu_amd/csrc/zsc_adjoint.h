@@ -443,6 +443,42 @@ __global__ void __launch_bounds__(64 * TY)
     const int nh = 2 * tw + 2 * (th + 2);  // ring cells per plane
     const int tid = ty * 64 + tx;
     const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
+    // Each thread loads at most one ring cell (nh <= 2 * 64 + 2 * (TY + 2)
+    // <= 64 * TY): its position in the plane and in LDS are the same for
+    // every plane, so they are worked out once.
+    static_assert(TY >= 3, "one ring cell per thread");
+    const bool ring = tid < nh;
+    int rr = 0, rc = 0;
+    uint32_t roff = 0;
+    if (ring) {
+        const int h = tid;
+        int hx, hy;
+        if (h < tw) {  // row below the tile
+            rc = h + 1;
+            rr = 0;
+            hx = x0 + h;
+            hy = y0 - 1;
+        } else if (h < 2 * tw) {  // row above
+            rc = h - tw + 1;
+            rr = th + 1;
+            hx = x0 + h - tw;
+            hy = y0 + th;
+        } else if (h < 2 * tw + th + 2) {  // column left, corners included
+            rc = 0;
+            rr = h - 2 * tw;
+            hx = x0 - 1;
+            hy = y0 - 1 + rr;
+        } else {  // column right
+            rc = tw + 1;
+            rr = h - 2 * tw - th - 2;
+            hx = x0 + tw;
+            hy = y0 - 1 + rr;
+        }
+        hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
+        hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+        roff = (uint32_t)hy * (uint32_t)nx + (uint32_t)hx;
+    }
+    const uint32_t ooff = (uint32_t)y * (uint32_t)nx + (uint32_t)x;
 
     // the four fields of plane zz (tile + ring) into its slot
     auto load_plane = [&](int zz) {
@@ -450,45 +486,20 @@ __global__ void __launch_bounds__(64 * TY)
         const int slot = (zz + 4) & 3;
         const uint32_t pz = (uint32_t)zc * nxy;
         if (own) {
-            const uint32_t o = pz + (uint32_t)y * (uint32_t)nx + (uint32_t)x;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) sh[slot][q][ty + 1][tx + 1] = ld32z<0>(fields + q * n, o);
+            for (int q = 0; q < 4; ++q)
+                sh[slot][q][ty + 1][tx + 1] = ld32z<0>(fields + q * n, pz + ooff);
         }
-        for (int h = tid; h < nh; h += 64 * TY) {
-            int hx, hy, r, col;
-            if (h < tw) {  // row below the tile
-                col = h + 1;
-                r = 0;
-                hx = x0 + h;
-                hy = y0 - 1;
-            } else if (h < 2 * tw) {  // row above
-                col = h - tw + 1;
-                r = th + 1;
-                hx = x0 + h - tw;
-                hy = y0 + th;
-            } else if (h < 2 * tw + th + 2) {  // column left, corners included
-                col = 0;
-                r = h - 2 * tw;
-                hx = x0 - 1;
-                hy = y0 - 1 + r;
-            } else {  // column right
-                col = tw + 1;
-                r = h - 2 * tw - th - 2;
-                hx = x0 + tw;
-                hy = y0 - 1 + r;
-            }
-            hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
-            hy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
-            const uint32_t o = pz + (uint32_t)hy * (uint32_t)nx + (uint32_t)hx;
+        if (ring) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) sh[slot][q][r][col] = ld32z<0>(fields + q * n, o);
+            for (int q = 0; q < 4; ++q) sh[slot][q][rr][rc] = ld32z<0>(fields + q * n, pz + roff);
         }
     };
 
     load_plane(z0 - 1);
     load_plane(z0);
     for (int k = z0; k < z1; ++k) {
-        const uint32_t o = (uint32_t)k * nxy + (uint32_t)y * (uint32_t)nx + (uint32_t)x;
+        const uint32_t o = (uint32_t)k * nxy + ooff;
         R lf[7];
         if (own && !DEFER) {
 #pragma unroll
