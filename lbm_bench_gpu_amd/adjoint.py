@@ -314,9 +314,15 @@ class AdjointRunner:
         """dL/dtheta from the tape, or by recomputing each checkpoint segment;
         lam = dL/da_N."""
         torch.cuda.current_stream().synchronize()
-        with torch.cuda.stream(self.stream):
-            g = self._backward(lam, scale)
-        self.stream.synchronize()
+        try:
+            with torch.cuda.stream(self.stream):
+                g = self._backward(lam, scale)
+        finally:
+            # also after an error: no recomputation may still write buffers
+            # that are freed with this call's
+            self.stream.synchronize()
+            if self.stream2 is not None:
+                self.stream2.synchronize()
         return g
 
     def _recompute(self, s0, s1, h=None, bank=None):
@@ -362,6 +368,9 @@ class AdjointRunner:
         h = self.ctx._h
         gp = ctypes.cast(self.grads.data_ptr(), ctypes.POINTER(ctypes.c_double))
         loss_slot = float(self.grads[14].item())
+        # drop coefficient sums a failed pass may have left in the scratch
+        # buffer (a no-op otherwise), so that this pass starts from none
+        check(lib().zsc_adj_flush(h, _vp(self.scratch), gp))
         self.grads.zero_()
         lam2, lam3 = self._lam(), self._lam()
         if self.tape is not None:

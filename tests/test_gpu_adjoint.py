@@ -181,6 +181,38 @@ def test_backward_steps_equal_single_steps(need_gpu, precision):
         assert torch.equal(out["single"][1], out["steps"][1])
 
 
+@pytest.mark.parametrize("overlap", [True, False])
+def test_backward_after_a_failed_pass(need_gpu, overlap):
+    """A backward pass stopped by a host-side error after its first segment
+    leaves that segment's coefficient sums in the scratch buffer; the next
+    pass drops them and gives the gradients of a pass that never failed,
+    bit for bit."""
+    import torch
+    dims = (14, 10, 8)
+    p, phi0 = small_problem(dims, seed=9)
+    theta = adjoint.theta_of(p)
+    run = adjoint.AdjointRunner(dims, p, precision="f32", checkpoint_every=4, tape=False,
+                                overlap=overlap)
+    tgt = run.forward([theta[0] * 1.1] + theta[1:], torch.from_numpy(phi0), 11,
+                      keep=False)[:7].sum(0).clone()
+    _, ref = run.loss_and_grad(theta, torch.from_numpy(phi0), tgt, 11)
+    a = run.forward(theta, torch.from_numpy(phi0), 11)
+    _, lam = run.loss(a, tgt)
+    steps = run._adjoint_steps
+
+    def failing(*args):
+        steps(*args)
+        raise RuntimeError("stopped after one segment")
+
+    run._adjoint_steps = failing
+    with pytest.raises(RuntimeError):
+        run.backward(lam.clone())
+    del run._adjoint_steps
+    g = run.backward(lam.clone())
+    print("after a failed pass", g, "\nreference        ", ref, flush=True)
+    assert g == ref
+
+
 def test_backward_steps_arguments(need_gpu):
     """Bad step counts, missing phi of a pulled state and aliased adjoint
     lattices are refused with ZSC_ERR_ARG before anything runs."""
